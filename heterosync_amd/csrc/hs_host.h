@@ -1,0 +1,66 @@
+// hs_host.h -- host-side internals shared by the library's translation
+// units: the context, error plumbing and small device-buffer helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/heterosync.h"
+
+struct hs_ctx {
+  int device = 0;
+  int num_cu = 0;
+  int max_threads = 0;
+  int max_wgs = 0;
+  double wall_hz = 0;  // wall_clock64() ticks per second
+  char arch[64] = {0};
+  hipStream_t stream = nullptr;
+};
+
+namespace hs {
+
+#define HS_HIP(call)                                                      \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) {                                               \
+      std::fprintf(stderr, "HIP error %s at %s:%d: %s\n", #call, __FILE__, \
+                   __LINE__, hipGetErrorString(e_));                      \
+      return HS_EHIP;                                                     \
+    }                                                                     \
+  } while (0)
+
+// Owns a set of device allocations; frees them on scope exit.
+class DevArena {
+ public:
+  ~DevArena() {
+    for (void *p : ptrs_) (void)hipFree(p);
+  }
+  template <typename T>
+  int alloc(T **out, size_t count) {
+    void *p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();
+      return HS_ENOMEM;
+    }
+    ptrs_.push_back(p);
+    *out = static_cast<T *>(p);
+    return HS_OK;
+  }
+
+ private:
+  std::vector<void *> ptrs_;
+};
+
+inline uint64_t timeout_ticks(const hs_ctx *c, double s) {
+  if (s <= 0) s = 20.0;
+  return (uint64_t)(s * c->wall_hz);
+}
+
+// Max co-resident WGs of `kernel` at `wg` threads per WG on this device.
+int coresident_wgs(const hs_ctx *c, const void *kernel, int wg, int *out);
+
+}  // namespace hs

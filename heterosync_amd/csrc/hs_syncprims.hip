@@ -1,0 +1,414 @@
+// hs_syncprims.hip -- the plain per-instance form of the syncPrims
+// microbenchmarks (hip/syncPrims/uvm/main.hip.cpp): one kernel launch per
+// repeat, each followed by a synchronize, the whole sequence timed as the
+// region of interest (main.hip.cpp:2354-2511).
+//
+// Kernels (one template instance per primitive family):
+//   k_mutex<KIND, UNIQ, MODE>   spin/EBO/sleep/FA mutex, global or per-CU
+//                               (main.hip.cpp:645-857)
+//   k_sem<EBO, UNIQ, MODE>      spin/EBO reader-writer semaphores
+//                               (main.hip.cpp:861-930, 1005-1073,
+//                                1147-1214, 1285-1352)
+//   k_barrier<FLAG, LX, MODE>   atomic / lock-free tree barrier, with and
+//                               without local exchange
+//                               (main.hip.cpp:104-149, 315-373, 542-643)
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "hs_host.h"
+#include "hs_syncprims_dev.h"
+
+using namespace hs;
+
+namespace {
+
+struct SPArgs {
+  int32_t *storage;
+  uint32_t *ctl;  // [0] abort word
+  uint32_t *locks;
+  uint32_t *ring;
+  long ring_len;
+  int ring_stride;
+  TreeBar tb;
+  FlagBar fb;
+  int L, iters, num_cu, nab, per_cu;
+  uint32_t e_global0, e_local0;
+  Backoff bo;
+  uint64_t ticks;
+};
+
+template <int KIND, bool UNIQ, int MODE>
+__global__ void __launch_bounds__(1024) k_mutex(SPArgs a) {
+  __shared__ int s_ok;
+  Guard g;
+  g.init(a.ctl, a.ticks);
+  const int lock = UNIQ ? (int)(blockIdx.x % a.num_cu) : 0;
+  uint32_t *lk = a.locks + (long)lock * kLockWords;
+  uint32_t *ring = a.ring ? a.ring + (long)lock * a.ring_len * a.ring_stride : nullptr;
+  // all WGs share region 0, or the per-CU region (main.hip.cpp:717, 742)
+  const long tid0 = UNIQ ? (long)lock * blockDim.x : 0;
+  uint32_t ticket = 0;
+  if (is_master()) s_ok = !g.aborted();
+  __syncthreads();
+  if (!s_ok) return;
+  for (int i = 0; i < a.iters; ++i) {
+    if (is_master()) s_ok = mutex_lock<KIND>(lk, ring, a.ring_stride, ticket, a.bo, g);
+    __syncthreads();
+    if (!s_ok) return;
+    wg_access<MODE>(a.storage, tid0, a.L);
+    __syncthreads();
+    if (is_master()) {
+      st_rlx(lk + kLkCs, ld_rlx(lk + kLkCs) + 1);  // plain, under the lock
+      mutex_unlock<KIND>(lk, ring, a.ring_stride, ticket);
+    }
+  }
+}
+
+template <bool EBO, bool UNIQ, int MODE>
+__global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
+  __shared__ int s_ok;
+  __shared__ int32_t sink[1024];
+  Guard g;
+  g.init(a.ctl, a.ticks);
+  const int b = blockIdx.x, grid = gridDim.x, nab = a.nab, per = a.per_cu;
+  const int cu = b % nab, pw = b / nab;
+  const bool writer = pw == cu % per;  // main.hip.cpp:883
+  uint32_t *lk = a.locks + (UNIQ ? (long)cu * kLockWords : 0);
+  sink[threadIdx.x] = 0;
+  if (is_master()) s_ok = !g.aborted();
+  __syncthreads();
+  if (!s_ok) return;
+  for (int i = 0; i < a.iters; ++i) {
+    if (is_master()) s_ok = sem_wait<EBO>(lk, writer, a.bo, g);
+    __syncthreads();
+    if (!s_ok) return;
+    if (writer) {
+      if (UNIQ) {  // main.hip.cpp:1040-1058: own region, then +numCU WGs
+        int cur = b;
+        for (int j = 0; j < per; ++j) {
+          wg_access<MODE>(a.storage, (long)cur * blockDim.x, a.L);
+          cur = (cur + nab) % grid;
+        }
+      } else {  // main.hip.cpp:903-915: regions 0..perCU-1
+        for (int j = 0; j < per; ++j)
+          wg_access<MODE>(a.storage, (long)j * blockDim.x, a.L);
+      }
+    } else {
+      wg_read(a.storage, (long)(UNIQ ? b : pw) * blockDim.x, a.L, sink);
+    }
+    __syncthreads();
+    if (is_master()) {
+      if (writer)
+        st_rlx(lk + kLkCs, ld_rlx(lk + kLkCs) + 1);  // exclusive
+      else
+        add_rlx(lk + kLkRd, 1);
+      s_ok = sem_post(lk, writer, a.bo, g);
+    }
+    __syncthreads();
+    if (!s_ok) return;
+  }
+}
+
+template <bool FLAG, bool LX, int MODE>
+__global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
+  __shared__ int s_ok, s_flag;
+  Guard g;
+  g.init(a.ctl, a.ticks);
+  const int b = blockIdx.x, grid = gridDim.x, nab = a.nab;
+  const int grp = b % nab, q = b / nab;  // cuID, perCU_WGID (HM:122-132)
+  int cur = b;
+  uint32_t eg = a.e_global0, el = a.e_local0;
+  if (is_master()) s_ok = !g.aborted();
+  __syncthreads();
+  if (!s_ok) return;
+  for (int i = 0; i < a.iters; ++i) {
+    wg_access<MODE>(a.storage, (long)cur * blockDim.x, a.L);
+    if (LX && a.per_cu > 1) {
+      bool ok;
+      if (FLAG) {
+        ok = flag_tree_barrier(a.fb, grp, q, ++el, 0, true, a.bo.poll_q, g, &s_flag);
+      } else {
+        __syncthreads();
+        if (is_master()) s_ok = atomic_tree_arrive(a.tb, grp, true, a.bo.poll_q, g);
+        __syncthreads();
+        ok = s_ok;
+      }
+      if (!ok) return;
+      cur = (cur + nab) % grid;  // HM:355-358
+      wg_access<MODE>(a.storage, (long)cur * blockDim.x, a.L);
+    }
+    bool ok;
+    if (FLAG) {
+      ok = flag_tree_barrier(a.fb, grp, q, ++el, ++eg, false, a.bo.poll_q, g, &s_flag);
+    } else {
+      __syncthreads();
+      if (is_master()) s_ok = atomic_tree_arrive(a.tb, grp, false, a.bo.poll_q, g);
+      __syncthreads();
+      ok = s_ok;
+    }
+    if (!ok) return;
+    cur = (cur + 1) % grid;  // HM:145-147
+  }
+}
+
+// Initialises lock blocks and (sleep mutex) ring slots.
+__global__ void k_init_locks(uint32_t *locks, int nlocks, uint32_t sem_max,
+                             uint32_t *ring, long ring_len, int stride) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nlocks && sem_max) {
+    uint32_t *lk = locks + i * kLockWords;
+    lk[kLkCount] = sem_max;
+    lk[kLkMax] = sem_max;
+    lk[kLkWaiting] = 1;  // HSS:23: the writer goes first
+  }
+  if (ring) {
+    const long total = (long)nlocks * ring_len;
+    for (long k = i; k < total; k += (long)gridDim.x * blockDim.x)
+      ring[k * stride] = (k % ring_len == 0) ? 1u : 0xffffffffu;  // HI:69-76
+  }
+}
+
+bool is_barrier(int p) { return p < 4; }
+bool is_mutex(int p) { return (p >= 4 && p <= 7) || (p >= 20 && p <= 23); }
+bool is_sem(int p) { return (p >= 8 && p <= 15) || (p >= 24 && p <= 31); }
+
+using KernelFn = void (*)(SPArgs);
+
+template <int MODE>
+KernelFn pick_kernel(int p) {
+  switch (p) {
+    case 0: return k_barrier<false, false, MODE>;
+    case 1: return k_barrier<false, true, MODE>;
+    case 2: return k_barrier<true, false, MODE>;
+    case 3: return k_barrier<true, true, MODE>;
+    case 4: return k_mutex<kSpin, false, MODE>;
+    case 5: return k_mutex<kEBO, false, MODE>;
+    case 6: return k_mutex<kSleep, false, MODE>;
+    case 7: return k_mutex<kFA, false, MODE>;
+    case 20: return k_mutex<kSpin, true, MODE>;
+    case 21: return k_mutex<kEBO, true, MODE>;
+    case 22: return k_mutex<kSleep, true, MODE>;
+    case 23: return k_mutex<kFA, true, MODE>;
+    default: break;
+  }
+  if (p >= 8 && p <= 11) return k_sem<false, false, MODE>;
+  if (p >= 12 && p <= 15) return k_sem<true, false, MODE>;
+  if (p >= 24 && p <= 27) return k_sem<false, true, MODE>;
+  if (p >= 28 && p <= 31) return k_sem<true, true, MODE>;
+  return nullptr;
+}
+
+uint32_t sem_count_of(int p) {
+  static const uint32_t kN[4] = {1, 2, 10, 120};  // spinSem1/2/10/120
+  if (p >= 8 && p <= 15) return kN[(p - 8) % 4];
+  if (p >= 24 && p <= 31) return kN[(p - 24) % 4];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
+                               int32_t *storage, long n, hs_syncprim_result *res) {
+  if (!c || !cfg || !storage || !res) return HS_EINVAL;
+  std::memset(res, 0, sizeof(*res));
+  const int p = cfg->prim;
+  if (hs_prim_name(p) == nullptr) return HS_EINVAL;
+  if (!hs_prim_supported(p)) return HS_EUNSUPPORTED;
+  const int L = cfg->num_ldst, nw = cfg->num_wgs, iters = cfg->num_cs_iters;
+  const int wg = cfg->wg_size ? cfg->wg_size : 64;
+  const int reps = cfg->reps ? cfg->reps : 10;
+  if (L < 0 || nw < 1 || nw > c->max_wgs || iters < 0 || reps < 1) return HS_EINVAL;
+  if (wg % 64 || wg > c->max_threads || wg > 1024) return HS_EINVAL;
+  if (cfg->mode != 0 && cfg->mode != 1) return HS_EINVAL;
+  const int ncu = c->num_cu;
+  const int nab = std::min(nw, ncu);
+  const int per = (nw + nab - 1) / nab;
+  const bool uniq = p >= 20;
+  if (is_barrier(p) && nw > ncu && nw % ncu) return HS_EINVAL;  // RD:80
+  if (is_sem(p) && uniq && nw % nab) return HS_EINVAL;  // each group needs its writer
+  if (n != hs_storage_words(p, nw, ncu, wg, L)) return HS_EINVAL;
+
+  KernelFn kfn = cfg->mode ? pick_kernel<1>(p) : pick_kernel<0>(p);
+  if (!kfn) return HS_EUNSUPPORTED;
+  HS_HIP(hipSetDevice(c->device));
+  if (is_barrier(p) || is_sem(p)) {  // forward progress needs co-residency
+    int fit = 0;
+    int rc = coresident_wgs(c, (const void *)kfn, wg, &fit);
+    if (rc) return rc;
+    if (fit < nw) return HS_EOCCUPANCY;
+  }
+
+  // ---- state, sized from the workload -------------------------------------
+  const int nlocks = uniq ? ncu : 1;
+  const bool sleep_mx = p == 6 || p == 22;
+  long ring_len = 0;
+  int ring_stride = 0;
+  if (sleep_mx) {
+    long tickets = uniq ? (long)((nw + ncu - 1) / ncu) : (long)nw;
+    tickets *= (long)iters * reps;
+    ring_len = tickets + 1;
+    // each slot on its own 128-byte line while the rings stay under 128 MiB
+    ring_stride = ((long)nlocks * ring_len <= (1L << 20)) ? HS_LINE_WORDS : 1;
+  }
+  DevArena arena;
+  int32_t *d_storage;
+  uint32_t *d_ctl, *d_locks, *d_ring = nullptr, *d_bar, *d_flags;
+  int rc;
+  if ((rc = arena.alloc(&d_storage, n))) return rc;
+  if ((rc = arena.alloc(&d_ctl, 64))) return rc;
+  if ((rc = arena.alloc(&d_locks, (size_t)nlocks * kLockWords))) return rc;
+  if (sleep_mx && (rc = arena.alloc(&d_ring, (size_t)nlocks * ring_len * ring_stride)))
+    return rc;
+  const size_t bar_words = (size_t)kBarWords * (1 + nab) + HS_LINE_WORDS;
+  if ((rc = arena.alloc(&d_bar, bar_words))) return rc;
+  const size_t flag_words = 2 * (size_t)nab + 2 * (size_t)nab * per + 4 * HS_LINE_WORDS;
+  if ((rc = arena.alloc(&d_flags, flag_words))) return rc;
+  HS_HIP(hipMemsetAsync(d_ctl, 0, 64 * sizeof(uint32_t), c->stream));
+  HS_HIP(hipMemsetAsync(d_locks, 0, (size_t)nlocks * kLockWords * 4, c->stream));
+  HS_HIP(hipMemsetAsync(d_bar, 0, bar_words * 4, c->stream));
+  HS_HIP(hipMemsetAsync(d_flags, 0, flag_words * 4, c->stream));
+  const uint32_t sem_max = sem_count_of(p) ? sem_count_of(p) + 1 : 0;  // count+1, HSS:15
+  if (sem_max || d_ring) {
+    long work = std::max<long>(nlocks, nlocks * ring_len);
+    int blocks = (int)std::min<long>((work + 255) / 256, 4096);
+    k_init_locks<<<blocks, 256, 0, c->stream>>>(d_locks, nlocks, sem_max, d_ring,
+                                                ring_len, ring_stride);
+    HS_HIP(hipGetLastError());
+  }
+  HS_HIP(hipMemcpyAsync(d_storage, storage, n * sizeof(int32_t), hipMemcpyHostToDevice,
+                        c->stream));
+  HS_HIP(hipStreamSynchronize(c->stream));
+
+  SPArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.storage = d_storage;
+  a.ctl = d_ctl;
+  a.locks = d_locks;
+  a.ring = d_ring;
+  a.ring_len = ring_len;
+  a.ring_stride = ring_stride;
+  uint32_t *episodes = d_bar + kBarWords * (1 + nab);
+  a.tb = TreeBar{d_bar, d_bar + kBarWords, episodes, nab, per};
+  a.fb = FlagBar{d_flags + 4 * HS_LINE_WORDS, d_flags + 4 * HS_LINE_WORDS + nab,
+                 d_flags + 4 * HS_LINE_WORDS + 2 * nab,
+                 d_flags + 4 * HS_LINE_WORDS + 2 * nab + (size_t)nab * per, episodes,
+                 nab, per};
+  a.L = L;
+  a.iters = iters;
+  a.num_cu = ncu;
+  a.nab = nab;
+  a.per_cu = per;
+  a.bo.min_q = cfg->backoff_min > 0 ? cfg->backoff_min : 1;
+  a.bo.max_q = cfg->backoff_max > 0 ? cfg->backoff_max : 1023;
+  a.bo.poll_q = cfg->poll_sleep < 0 ? 0 : (cfg->poll_sleep > 0 ? cfg->poll_sleep : 1);
+  a.ticks = timeout_ticks(c, cfg->timeout_s);
+  const bool lx = p == 1 || p == 3;
+  const uint32_t local_per_iter = (lx && per > 1) ? 2 : 1;
+
+  // ---- region of interest (main.hip.cpp:2354-2511) -----------------------
+  hipEvent_t ev0, ev1;
+  HS_HIP(hipEventCreate(&ev0));
+  HS_HIP(hipEventCreate(&ev1));
+  std::vector<hipEvent_t> kev(2 * reps);
+  for (auto &e : kev) HS_HIP(hipEventCreate(&e));
+  HS_HIP(hipDeviceSynchronize());
+  HS_HIP(hipEventRecord(ev0, c->stream));
+  int launch_rc = HS_OK;
+  for (int r = 0; r < reps; ++r) {
+    a.e_global0 = (uint32_t)r * iters;
+    a.e_local0 = (uint32_t)r * iters * local_per_iter;
+    HS_HIP(hipEventRecord(kev[2 * r], c->stream));
+    hipLaunchKernelGGL(kfn, dim3(nw), dim3(wg), 0, c->stream, a);
+    if (hipGetLastError() != hipSuccess) { launch_rc = HS_EHIP; break; }
+    HS_HIP(hipEventRecord(kev[2 * r + 1], c->stream));
+    HS_HIP(hipStreamSynchronize(c->stream));
+  }
+  HS_HIP(hipEventRecord(ev1, c->stream));
+  HS_HIP(hipEventSynchronize(ev1));
+  float ms = 0;
+  HS_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  res->roi_ms = ms;
+  double kms = 0;
+  for (int r = 0; r < reps && launch_rc == HS_OK; ++r) {
+    float k = 0;
+    HS_HIP(hipEventElapsedTime(&k, kev[2 * r], kev[2 * r + 1]));
+    kms += k;
+  }
+  res->kernel_ms = kms;
+  for (auto &e : kev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  if (launch_rc) return launch_rc;
+
+  // ---- results and idle-state invariants ----------------------------------
+  HS_HIP(hipMemcpy(storage, d_storage, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  uint32_t abort_w = 0;
+  HS_HIP(hipMemcpy(&abort_w, d_ctl, 4, hipMemcpyDeviceToHost));
+  std::vector<uint32_t> locks((size_t)nlocks * kLockWords), bar(bar_words), flags(flag_words);
+  HS_HIP(hipMemcpy(locks.data(), d_locks, locks.size() * 4, hipMemcpyDeviceToHost));
+  HS_HIP(hipMemcpy(bar.data(), d_bar, bar.size() * 4, hipMemcpyDeviceToHost));
+  HS_HIP(hipMemcpy(flags.data(), d_flags, flags.size() * 4, hipMemcpyDeviceToHost));
+  const uint64_t total_ep = (uint64_t)iters * reps;
+  bool ok = true;
+  if (is_mutex(p) || is_sem(p)) {
+    for (int l = 0; l < nlocks; ++l) {
+      const uint32_t *lk = &locks[(size_t)l * kLockWords];
+      res->cs_entries += lk[kLkCs];
+      res->rd_entries += lk[kLkRd];
+      // per-lock acquisitions: WGs mapped to this lock x iters x reps
+      long wgs_here = uniq ? (nw / ncu + (l < nw % ncu ? 1 : 0)) : nw;
+      if (is_sem(p) && uniq) wgs_here = (l < nab) ? per : 0;
+      const uint64_t acq = (uint64_t)wgs_here * iters * reps;
+      if (p == 4 || p == 5 || p == 20 || p == 21) ok = ok && lk[kLkHead] == 0;
+      if (p == 7 || p == 23) ok = ok && lk[kLkHead] == (uint32_t)acq && lk[kLkTail] == (uint32_t)acq;
+      if (sleep_mx) {
+        ok = ok && lk[kLkTail] == (uint32_t)acq;
+        if (acq + 1 <= (uint64_t)ring_len) {
+          std::vector<uint32_t> ring(ring_len * ring_stride);
+          HS_HIP(hipMemcpy(ring.data(), d_ring + (size_t)l * ring_len * ring_stride,
+                           ring.size() * 4, hipMemcpyDeviceToHost));
+          for (long k = 0; k < ring_len; ++k)
+            ok = ok && ring[k * ring_stride] == ((uint64_t)k == acq ? 1u : 0xffffffffu);
+        } else {
+          ok = false;
+        }
+      }
+      if (is_sem(p) && (!uniq || l < nab))
+        ok = ok && lk[kLkHead] == 0 && lk[kLkCount] == sem_max && lk[kLkWaiting] == 0;
+    }
+    const uint32_t *lk0 = &locks[0];
+    const uint32_t w[8] = {lk0[kLkHead], lk0[kLkCount], lk0[kLkWaiting], lk0[kLkMax],
+                           lk0[kLkTail], lk0[kLkCs], lk0[kLkRd], (uint32_t)nlocks};
+    std::memcpy(res->lock_words, w, sizeof(w));
+  } else {
+    const uint32_t ep = bar[kBarWords * (1 + nab)];
+    res->episodes = ep;
+    ok = ok && ep == (uint32_t)total_ep;
+    if (p <= 1) {
+      ok = ok && bar[0] == 0 && bar[32] == (uint32_t)total_ep;
+      if (per > 1)
+        for (int gi = 0; gi < nab; ++gi)
+          ok = ok && bar[kBarWords * (1 + gi)] == 0 &&
+               bar[kBarWords * (1 + gi) + 32] == (uint32_t)(total_ep * local_per_iter);
+      const uint32_t w[8] = {bar[0], bar[32], bar[kBarWords], bar[kBarWords + 32], ep, 0, 0, 0};
+      std::memcpy(res->lock_words, w, sizeof(w));
+    } else {
+      const uint32_t *gin = &flags[4 * HS_LINE_WORDS], *gout = gin + nab;
+      const uint32_t *lin = gout + nab, *lout = lin + (size_t)nab * per;
+      for (int gi = 1; gi < nab; ++gi)
+        ok = ok && gin[gi] == (uint32_t)total_ep && gout[gi] == (uint32_t)total_ep;
+      if (per > 1)
+        for (int gi = 0; gi < nab; ++gi)
+          for (int m = 1; m < per; ++m)
+            ok = ok && lin[(size_t)gi * per + m] == (uint32_t)(total_ep * local_per_iter) &&
+                 lout[(size_t)gi * per + m] == (uint32_t)(total_ep * local_per_iter);
+      const uint32_t w[8] = {nab > 1 ? gin[1] : 0, nab > 1 ? gout[1] : 0,
+                             per > 1 ? lin[1] : 0, per > 1 ? lout[1] : 0, ep, 0, 0, 0};
+      std::memcpy(res->lock_words, w, sizeof(w));
+    }
+  }
+  res->state_ok = ok ? 1 : 0;
+  if (abort_w) return HS_ETIMEOUT;
+  return HS_OK;
+}

@@ -1,0 +1,250 @@
+/*
+ * heterosync.h -- C ABI of the MI355X (gfx950) synchronization hot path.
+ *
+ * One shared library, libheterosync_amd.so, built by hipcc.  Plain C types
+ * only: host pointers and sizes in, results out.  Every function returns
+ * HS_OK (0) or a negative HS_E* code; hs_strerror() gives the text.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   hs_open / hs_close         hipLocksInit / hipLocksDestroy
+ *                              (hip/syncPrims/uvm/hipLocksImpl.h:3-93, :95-108)
+ *                              plus the device query of
+ *                              hip/syncPrims/uvm/main.hip.cpp:1995-2011
+ *   hs_prim_id / hs_prim_name  the string->id table main.hip.cpp:2036-2089
+ *   hs_storage_words           numLocsMult/numStorageLocs main.hip.cpp:2109-2136
+ *   hs_syncprim_run            hipMutexCreate* / hipSemaphoreCreate* +
+ *                              the ROI and invoke* launchers
+ *                              main.hip.cpp:1423-1927, :2354-2511
+ *   hs_seqlock_run             hip/relaxedAtomics/uvm/seqlocks/seqlocks.cpp:6-112
+ *                              (+ seqlocks_hipAtomics/seqlocks.cpp)
+ *   hs_flags_run               cuda/relaxedAtomics/uvm/flags/flags.cu:19-176
+ *   hs_refcounter_run          cuda/relaxedAtomics/uvm/refCounter/refCounter.cu:5-155
+ *   hs_splitcounter_run        cuda/relaxedAtomics/uvm/splitCounter/splitCounter.cu:5-93
+ *   hs_uts_ref_run             cuda/uts/uvm/uts.cu:244-455 (callGPU/doWork)
+ *   hs_uts_geo_run             no reference implementation (SHA-1 UTS,
+ *                              the public UTS algorithm; see hs_uts_geo_run)
+ *   hs_batch_run, hs_uts_serve extensions with no reference counterpart:
+ *                              batched persistent syncPrims and the UTS
+ *                              serving loop.
+ */
+#ifndef HETEROSYNC_H
+#define HETEROSYNC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  HS_OK = 0,
+  HS_EINVAL = -1,      /* bad argument / unsupported shape (usage error) */
+  HS_EHIP = -2,        /* a HIP runtime call failed */
+  HS_ENOMEM = -3,      /* device allocation failed */
+  HS_EOCCUPANCY = -4,  /* workgroups cannot all be co-resident */
+  HS_ETIMEOUT = -5,    /* device watchdog fired: a wait exceeded timeout_s */
+  HS_ENODEV = -6,      /* no HIP device */
+  HS_EUNSUPPORTED = -7 /* primitive id exists in the reference table but is
+                          not part of this build (SRB, priority semaphores) */
+};
+
+const char *hs_strerror(int code);
+
+typedef struct hs_ctx hs_ctx;
+
+/* Opens device `device` (hipSetDevice) and queries it. */
+int hs_open(int device, hs_ctx **out);
+void hs_close(hs_ctx *ctx);
+
+/* NUM_CU, maxThreadsPerBlock, MAX_WGS = (maxThreadsPerBlock/64)*NUM_CU
+ * (main.hip.cpp:2004-2008), gcnArchName. */
+int hs_device_info(hs_ctx *ctx, int *num_cu, int *max_threads_per_block,
+                   int *max_wgs, char *arch, int arch_len);
+
+/* ---- syncPrims -------------------------------------------------------- */
+
+/* Primitive ids follow main.hip.cpp:2036-2089 exactly (0..42, 36 and
+ * 43-46 reserved).  Returns -1 for unknown names. */
+int hs_prim_id(const char *name);
+const char *hs_prim_name(int id);
+/* 1 if this build implements the id (0-15, 20-31), else 0. */
+int hs_prim_supported(int id);
+
+/* Number of int32 storage words a run uses (main.hip.cpp:2109-2136 with
+ * NUM_WIS_PER_WG = wg_size).  Negative on error. */
+long hs_storage_words(int prim, int num_wgs, int num_cu, int wg_size,
+                      int num_ldst);
+
+typedef struct {
+  int prim;          /* hs_prim_id() value */
+  int num_ldst;      /* argv[2] of allSyncPrims-1kernel */
+  int num_wgs;       /* argv[3] */
+  int num_cs_iters;  /* argv[4] */
+  int wg_size;       /* threads per WG; 0 = 64 (main.hip.cpp:9) */
+  int reps;          /* launches per run; 0 = 10 (main.hip.cpp:21) */
+  int mode;          /* 0 = copy chain (reference), 1 = in-place increment */
+  int backoff_min;   /* EBO backoff start, s_sleep quanta; 0 = 1 */
+  int backoff_max;   /* EBO cap mask (2^k-1); 0 = 1023 (MAX_BACKOFF-1) */
+  int poll_sleep;    /* quanta between plain polls; <0 = 0, 0 = default */
+  double timeout_s;  /* per-launch watchdog; 0 = 20 s */
+} hs_syncprim_cfg;
+
+typedef struct {
+  double roi_ms;        /* region of interest: all launches + syncs */
+  double kernel_ms;     /* sum of per-launch kernel times (HIP events) */
+  int64_t cs_entries;   /* exclusive entries (mutex CS / semaphore writers) */
+  int64_t rd_entries;   /* semaphore reader entries */
+  int64_t episodes;     /* completed barrier episodes */
+  int state_ok;         /* 1 if every lock/barrier word is back in its
+                           idle state (see hs_syncprim_run) */
+  uint32_t lock_words[8]; /* first lock block / barrier words, for tests */
+} hs_syncprim_result;
+
+/* Runs one microbenchmark exactly as the reference's ROI does: `reps`
+ * launches of numWGs x wg_size, each followed by a device synchronize.
+ * storage: host array of hs_storage_words() int32 words; copied to the
+ * device before the ROI and back after it (not timed).  Lock and barrier
+ * state is created per call, sized from the workload (no fixed worst-case
+ * buffers).  Barriers and semaphores need every WG co-resident and return
+ * HS_EOCCUPANCY otherwise. */
+int hs_syncprim_run(hs_ctx *ctx, const hs_syncprim_cfg *cfg, int32_t *storage,
+                    long n_words, hs_syncprim_result *res);
+
+/* ---- relaxed atomics ---------------------------------------------------- */
+
+typedef struct {
+  double kernel_ms;
+  uint64_t atomic_ops;  /* device-scope atomic RMWs/loads/stores issued on
+                           sync words (algorithmic count) */
+  int aborted;
+} hs_relaxed_result;
+
+/* seqlock: writer WGs are b % writer_period == 0 (reference: 16), lock
+ * index b % group_size, `iters` passes per WG (reference: 10).  variant 0 =
+ * scoped acquire/release atomics (seqlocks_hipAtomics), 1 = relaxed atomics
+ * plus fences with exchanges as loads (seqlocks, useTFs).  seq_out has
+ * n_seq >= group_size words; out_arr has num_wgs*wg_size words;
+ * torn_reads counts reader snapshots whose two data words differed
+ * (variant 0 only; must be 0). */
+int hs_seqlock_run(hs_ctx *ctx, int num_wgs, int wg_size, int group_size,
+                   int writer_period, int iters, int variant,
+                   uint32_t *seq_out, int n_seq, int32_t *out_arr,
+                   uint64_t *torn_reads, hs_relaxed_result *res);
+
+/* flags: the last TB is the main TB; workers loop num_iters times (and
+ * until stop is seen), bumping `dirty` when rand[b + it*num_tbs] <= cutoff.
+ * rand_arr has num_tbs*num_iters floats; out_arr num_tbs*tb_size words. */
+int hs_flags_run(hs_ctx *ctx, int num_tbs, int tb_size, int num_accs_per_thr,
+                 int num_repeats, float cutoff, int num_iters,
+                 const float *rand_arr, uint32_t *out_arr, uint32_t *dirty,
+                 hs_relaxed_result *res);
+
+/* refCounter: num_repeats x {+1 on c0,c1; `spacing` LDS steps;
+ * atomicDec(c, 1e9) on c0, c1; del = old <= 1}.  spacing < 0 = the
+ * reference's 2*num_repeats.  c0/c1 have num_tbs*tb_size/sharers words,
+ * del0/del1 num_tbs*tb_size. */
+int hs_refcounter_run(hs_ctx *ctx, int sharers, int num_tbs, int tb_size,
+                      int num_repeats, int spacing, uint32_t *c0,
+                      uint32_t *c1, uint32_t *del0, uint32_t *del1,
+                      hs_relaxed_result *res);
+
+/* splitCounter: per wave, lanes 0-31 are writers (one atomicAdd(+1) per
+ * iteration on their own counter), lanes 32-63 are readers (sum of all
+ * counters of the reader's CU partition plus the global approximate total).
+ * counters: num_tbs*tb_size/2 words; out_arr num_tbs*tb_size words. */
+int hs_splitcounter_run(hs_ctx *ctx, int num_tbs, int tb_size, int iters,
+                        uint32_t *counters, uint32_t *out_arr,
+                        hs_relaxed_result *res);
+
+/* ---- UTS ----------------------------------------------------------------- */
+
+typedef struct {
+  double kernel_ms;
+  uint64_t nodes;       /* nodes processed (device count) */
+  uint64_t leaves;
+  uint64_t leaf_sum;    /* reference tree only */
+  int max_depth;        /* geometric trees only */
+  int aborted;
+} hs_uts_result;
+
+/* Reference binary tree (uts.cu:290-339 layout: value, child_a, child_b per
+ * node; child_a == 0 means leaf).  Work-stealing over per-WG deques in
+ * HBM; num_blocks WGs of 64 lanes (0 = 2 per CU). */
+int hs_uts_ref_run(hs_ctx *ctx, int num_nodes, const uint32_t *value,
+                   const int32_t *child_a, const int32_t *child_b,
+                   int num_blocks, hs_uts_result *res);
+
+/* One geometric (fixed shape) SHA-1 tree: root seed, expected branching
+ * b0, depth limit gen_mx (UTS -t 1 -a 3 -b b0 -d gen_mx -r seed). */
+int hs_uts_geo_run(hs_ctx *ctx, int seed, double b0, int gen_mx,
+                   int num_blocks, hs_uts_result *res);
+
+/* ---- extensions ---------------------------------------------------------- */
+
+typedef struct {
+  int prim;       /* mutex or semaphore id (4-15) */
+  int num_wgs;
+  int num_cs_iters;
+  int num_ldst;
+} hs_batch_instance;
+
+typedef struct {
+  int wg_size;       /* 0 = 64 */
+  int reps;          /* 0 = 10 */
+  int mode;          /* 0 copy, 1 inc */
+  int backoff_min, backoff_max, poll_sleep;
+  int use_graph;     /* capture the launches in a hipGraph */
+  int persistent_wgs; /* workers of the persistent grid; 0 = auto */
+  double timeout_s;
+} hs_batch_cfg;
+
+typedef struct {
+  double roi_ms;
+  double kernel_ms;
+  int64_t acquires;   /* total acquire+release pairs (sum over instances) */
+  int all_ok;         /* every instance's state/counters matched */
+} hs_batch_result;
+
+/* Runs n instances in one persistent launch per rep.  storages[i] is the
+ * host array of instance i (hs_storage_words(prim,num_wgs,...) words);
+ * entries[i] receives instance i's counted CS entries. */
+int hs_batch_run(hs_ctx *ctx, const hs_batch_instance *inst, int n,
+                 const hs_batch_cfg *cfg, int32_t *const *storages,
+                 int64_t *entries, hs_batch_result *res);
+
+typedef struct {
+  int seed;
+  int gen_mx;
+  double b0;
+  double arrival_s;   /* offered arrival time relative to start */
+} hs_uts_request;
+
+typedef struct {
+  uint64_t nodes, leaves;
+  int max_depth;
+  double admit_s, done_s;  /* device-observed admission / completion */
+} hs_uts_response;
+
+typedef struct {
+  int slots;          /* concurrent trees on the device; 0 = 8 */
+  int num_blocks;     /* persistent workers; 0 = 2 per CU */
+  int deque_nodes;    /* per-worker deque capacity (nodes); 0 = auto */
+  double timeout_s;
+} hs_serve_cfg;
+
+typedef struct {
+  double wall_s;
+  uint64_t total_nodes;
+  double p50_s, p99_s;
+} hs_serve_summary;
+
+int hs_uts_serve(hs_ctx *ctx, const hs_uts_request *req, int n,
+                 const hs_serve_cfg *cfg, hs_uts_response *resp,
+                 hs_serve_summary *sum);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,360 @@
+"""CPU tests that pin the oracle (oracle/hs_oracle.c, oracle/cpu_emul.cpp).
+
+Each function of the oracle is checked against something it does not
+restate: published known-answer vectors, an independent implementation
+(hashlib, a Python glibc-rand restatement), brute force over the kernels'
+access pattern, or a real concurrent run (std::thread emulation).  Planted
+mistakes (a wrong pass count, a dropped line) must be caught.
+"""
+import ctypes
+import hashlib
+import math
+import random
+
+import numpy as np
+import pytest
+
+P8 = ctypes.POINTER(ctypes.c_uint8)
+P32 = ctypes.POINTER(ctypes.c_int32)
+
+SUPPORTED = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15,
+             20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31]
+
+REF_NAMES = {  # hip/syncPrims/uvm/main.hip.cpp:2036-2089
+    "atomicTreeBarrUniq": 0, "atomicTreeBarrUniqLocalExch": 1, "lfTreeBarrUniq": 2,
+    "lfTreeBarrUniqLocalExch": 3, "spinMutex": 4, "spinMutexEBO": 5, "sleepMutex": 6,
+    "faMutex": 7, "spinSem1": 8, "spinSem2": 9, "spinSem10": 10, "spinSem120": 11,
+    "spinSemEBO1": 12, "spinSemEBO2": 13, "spinSemEBO10": 14, "spinSemEBO120": 15,
+    "PRspinSem1": 16, "PRspinSem2": 17, "PRspinSem10": 18, "PRspinSem120": 19,
+    "spinMutexUniq": 20, "spinMutexEBOUniq": 21, "sleepMutexUniq": 22, "faMutexUniq": 23,
+    "spinSemUniq1": 24, "spinSemUniq2": 25, "spinSemUniq10": 26, "spinSemUniq120": 27,
+    "spinSemEBOUniq1": 28, "spinSemEBOUniq2": 29, "spinSemEBOUniq10": 30,
+    "spinSemEBOUniq120": 31, "PRspinSemEBO1": 32, "PRspinSemEBO2": 33,
+    "PRspinSemEBO10": 34, "PRspinSemEBO120": 35, "atomicTreeBarrUniqSRB": 37,
+    "atomicTreeBarrUniqLocalExchSRB": 38, "atomicTreeBarrUniqNaiveSRB": 39,
+    "atomicTreeBarrUniqLocalExchNaiveSRB": 40, "atomicTreeBarrUniqNaiveAllSRB": 41,
+    "atomicTreeBarrUniqLocalExchNaiveAllSRB": 42,
+}
+
+
+def sha1(oracle, msg: bytes) -> bytes:
+    out = (ctypes.c_uint8 * 20)()
+    oracle.hso_sha1(msg, len(msg), out)
+    return bytes(out)
+
+
+# ---- SHA-1 (FIPS 180-4 known answers) --------------------------------------
+
+@pytest.mark.parametrize("msg,digest", [
+    (b"abc", "a9993e364706816aba3e25717850c26c9cd0d89d"),
+    (b"", "da39a3ee5e6b4b0d3255bfef95601890afd80709"),
+    (b"abcdbcdecdefdefgefghfghighijhijkijkljklmklmnlmnomnopnopq",
+     "84983e441c3bd26ebaae4aa1f95129e5e54670f1"),
+    (b"a" * 1000000, "34aa973cd4c4daa4f61eeb2bdbad27316534016f"),
+])
+def test_sha1_known_answers(oracle, msg, digest):
+    assert sha1(oracle, msg).hex() == digest
+
+
+def test_sha1_matches_hashlib_all_lengths(oracle):
+    rng = random.Random(7)
+    for n in range(0, 200):
+        m = bytes(rng.randrange(256) for _ in range(n))
+        assert sha1(oracle, m) == hashlib.sha1(m).digest(), n
+
+
+# ---- UTS geometric trees ---------------------------------------------------
+
+def test_uts_rng_definition(oracle):
+    """Root = SHA1(16 zero bytes || seed BE32); child i = SHA1(parent || i BE32)."""
+    root = (ctypes.c_uint8 * 20)()
+    oracle.hso_uts_root(19, root)
+    assert bytes(root) == hashlib.sha1(bytes(16) + (19).to_bytes(4, "big")).digest()
+    child = (ctypes.c_uint8 * 20)()
+    oracle.hso_uts_spawn(root, 3, child)
+    assert bytes(child) == hashlib.sha1(bytes(root) + (3).to_bytes(4, "big")).digest()
+    want = int.from_bytes(bytes(child)[16:20], "big") & 0x7FFFFFFF
+    assert oracle.hso_uts_rand(child) == want
+
+
+def test_uts_T1_published_sample_tree(oracle):
+    """UTS sample tree T1 (-t 1 -a 3 -d 10 -b 4 -r 19): 4130071 nodes,
+    3305118 leaves, depth 10 -- the public UTS distribution's figures."""
+    n, lv, md = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+    oracle.hso_uts_geo_count(19, 4.0, 10, ctypes.byref(n), ctypes.byref(lv), ctypes.byref(md))
+    assert (n.value, lv.value, md.value) == (4130071, 3305118, 10)
+
+
+def _py_children(h, b0, depth, gen_mx):
+    b = b0 if (depth < gen_mx or depth == 0) else 0.0
+    p = 1.0 / (1.0 + b)
+    u = h / 2147483648.0
+    if p >= 1.0:
+        return 0
+    return min(100, int(math.floor(math.log(1.0 - u) / math.log(1.0 - p))))
+
+
+def test_uts_geo_bruteforce_small_trees(oracle):
+    """Exhaustive count against an independent hashlib-based recursion."""
+    for seed, depth in [(0, 3), (1, 4), (19, 4), (42, 5), (7, 2)]:
+        def walk(st, d):
+            h = int.from_bytes(st[16:20], "big") & 0x7FFFFFFF
+            nc = _py_children(h, 4.0, d, depth)
+            if nc == 0:
+                return 1, 1, d
+            n, l, m = 1, 0, d
+            for i in range(nc):
+                c = hashlib.sha1(st + i.to_bytes(4, "big")).digest()
+                a, b, e = walk(c, d + 1)
+                n, l, m = n + a, l + b, max(m, e)
+            return n, l, m
+        root = hashlib.sha1(bytes(16) + seed.to_bytes(4, "big")).digest()
+        want = walk(root, 0)
+        n, lv, md = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+        oracle.hso_uts_geo_count(seed, 4.0, depth, ctypes.byref(n), ctypes.byref(lv),
+                                 ctypes.byref(md))
+        assert (n.value, lv.value, md.value) == want
+
+
+def test_uts_integer_thresholds_equal_float_formula(oracle):
+    thr = (ctypes.c_uint32 * 128)()
+    finite = oracle.hso_uts_geo_thresholds(4.0, thr, 128)
+    assert 90 <= finite <= 100
+    t = list(thr)[:finite]
+    assert t == sorted(t) and t[0] > 0
+    rng = random.Random(3)
+    samples = [rng.randrange(1 << 31) for _ in range(20000)]
+    for x in t:  # every boundary and its neighbours
+        samples += [x - 1, x, x + 1]
+    for h in samples:
+        if not 0 <= h < (1 << 31):
+            continue
+        by_thr = sum(1 for x in t if h >= x)
+        assert by_thr == _py_children(h, 4.0, 0, 10), h
+    # closed form of the first threshold: 2^31 * (1 - 0.8)
+    assert abs(t[0] - (1 << 31) * 0.2) <= 1
+
+
+# ---- glibc rand (flags input) ----------------------------------------------
+
+def _glibc_rand(seed, n):
+    """The published glibc TYPE_3 additive-feedback generator (random_r.c)."""
+    r = [0] * 34
+    r[0] = seed if seed else 1
+    for i in range(1, 31):
+        hi, lo = divmod(r[i - 1], 127773)
+        word = 16807 * lo - 2836 * hi
+        if word < 0:
+            word += 2147483647
+        r[i] = word
+    for i in range(31, 34):
+        r[i] = r[i - 31]
+    state = r[:]
+    out = []
+    k = 34
+    while len(out) < n + 310:
+        v = (state[k - 31] + state[k - 3]) & 0xFFFFFFFF
+        state.append(v)
+        k += 1
+        if k - 1 >= 344:
+            out.append(v >> 1)
+    return out[:n]
+
+
+def test_flags_rand_is_glibc(oracle):
+    n = 1000
+    arr = np.zeros(n, np.float32)
+    oracle.hso_flags_rand(arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n, 2018)
+    want = np.array([np.float32(v) / np.float32(4294967295.0) for v in _glibc_rand(2018, n)],
+                    np.float32)
+    assert np.array_equal(arr, want)
+    assert arr.max() <= 0.5  # rand() <= 2^31-1 -> values in [0, 0.5]
+
+
+def test_flags_dirty_count_bruteforce(oracle):
+    ntb, iters = 7, 13
+    arr = np.zeros(ntb * iters, np.float32)
+    oracle.hso_flags_rand(arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), arr.size, 2018)
+    want = sum(1 for b in range(ntb - 1) for it in range(iters) if arr[b + it * ntb] <= 0.33)
+    got = oracle.hso_flags_dirty(arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ntb,
+                                 iters, ctypes.c_float(0.33))
+    assert got == want
+
+
+# ---- seqlock ------------------------------------------------------------------
+
+@pytest.mark.parametrize("nw,gs,period,iters", [(512, 4, 16, 10), (1024, 16, 65, 10000),
+                                                (33, 5, 3, 7)])
+def test_seqlock_expected_bruteforce(oracle, nw, gs, period, iters):
+    seq = np.zeros(gs, np.uint32)
+    oracle.hso_seqlock_expected(nw, gs, period, iters,
+                                seq.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), gs)
+    want = np.zeros(gs, np.uint64)
+    for b in range(nw):
+        if b % 16 == 0 if period == 16 else b % period == 0:
+            for _ in range(iters):
+                want[b % gs] += 2
+    assert np.array_equal(seq, want.astype(np.uint32))
+    assert all(int(s) % 2 == 0 for s in seq)
+
+
+# ---- syncPrims goldens -----------------------------------------------------
+
+def test_prim_table_matches_reference(oracle):
+    for name, pid in REF_NAMES.items():
+        assert oracle.hso_prim_id(name.encode()) == pid
+    assert oracle.hso_prim_id(b"noSuchPrim") == -1
+
+
+def _simulate(prim, L, nw, iters, reps, wg, ncu, mode):
+    """Brute force: replay the kernels' per-WG region walk, one pass at a time."""
+    n = None
+    if prim < 4 or 24 <= prim <= 31:
+        mult = nw
+    elif prim <= 7:
+        mult = 1
+    elif prim <= 15:
+        mult = -(-nw // ncu)
+    else:
+        mult = min(nw, ncu)
+    n = mult * wg * (L + 1)
+    s = np.arange(n, dtype=np.int64)
+    nab = min(nw, ncu)
+    per = -(-nw // nab)
+
+    def one_pass(region):
+        for t in range(wg):
+            tid = region * wg + t
+            base = (tid // 16) * (L + 1)
+            off = t % 16
+            for k in range(L - 1, -1, -1):
+                dst = (base + k + 1) * 16 + off
+                if mode == 0:
+                    s[dst] = s[(base + k) * 16 + off]
+                else:
+                    s[dst] += 1
+
+    for _ in range(reps):
+        for i in range(iters):
+            if prim < 4:
+                for b in range(nw):  # episode i: WG b at region (b + shift)
+                    cur = (b + i * ((nab + 1) if (prim in (1, 3) and per > 1) else 1)) % nw
+                    one_pass(cur)
+                    if prim in (1, 3) and per > 1:
+                        one_pass((cur + nab) % nw)
+            elif prim <= 7:
+                for b in range(nw):
+                    one_pass(0)
+            elif prim <= 15:
+                for b in range(nw):
+                    if b // nab == (b % nab) % per:
+                        for j in range(per):
+                            one_pass(j)
+            elif prim <= 23:
+                for b in range(nw):
+                    one_pass(b % ncu)
+            else:
+                for b in range(nw):
+                    if b // nab == (b % nab) % per:
+                        cur = b
+                        for j in range(per):
+                            one_pass(cur)
+                            cur = (cur + nab) % nw
+    return s.astype(np.int32)
+
+
+SHAPES = [(4, 4), (8, 4), (2, 4), (12, 4), (16, 8)]
+
+
+@pytest.mark.parametrize("prim", SUPPORTED)
+def test_literal_golden_equals_closed_form(ohelp, prim):
+    for nw, ncu in SHAPES:
+        for L in (0, 1, 3):
+            for mode in (0, 1):
+                try:
+                    closed = ohelp.golden(prim, L, nw, 3, 2, 64, ncu, mode)
+                except AssertionError:
+                    continue  # shape the primitive does not accept
+                lit = ohelp.golden(prim, L, nw, 3, 2, 64, ncu, mode, literal=True)
+                assert np.array_equal(lit, closed), (prim, nw, ncu, L, mode)
+
+
+@pytest.mark.parametrize("prim", SUPPORTED)
+def test_closed_form_equals_bruteforce_walk(ohelp, prim):
+    for nw, ncu in [(4, 4), (8, 4), (12, 4), (3, 4)]:
+        for mode in (0, 1):
+            try:
+                closed = ohelp.golden(prim, 2, nw, 2, 2, 64, ncu, mode)
+            except AssertionError:
+                continue
+            sim = _simulate(prim, 2, nw, 2, 2, 64, ncu, mode)
+            assert np.array_equal(sim, closed), (prim, nw, ncu, mode)
+
+
+def test_planted_mistakes_are_caught(ohelp):
+    """A wrong pass count or a dropped line changes the expected state."""
+    good = ohelp.golden(7, 3, 4, 1, 1, 64, 4, 1)  # inc mode, k = 4
+    lit = ohelp.golden(7, 3, 4, 1, 1, 64, 4, 1, literal=True)
+    assert np.array_equal(good, lit)
+    off_by_one = ohelp.golden(7, 3, 3, 1, 1, 64, 4, 1)  # one WG fewer
+    assert not np.array_equal(good, off_by_one)
+    dropped = good.copy()
+    dropped[16 * 2 + 5] -= 1
+    assert not np.array_equal(dropped, lit)
+    # copy mode with k < L is sensitive to k as well
+    a = ohelp.golden(0, 10, 4, 3, 1, 64, 4, 0)
+    b = ohelp.golden(0, 10, 4, 4, 1, 64, 4, 0)
+    assert not np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prim", SUPPORTED)
+def test_cpu_emulation_matches_closed_form(oracle, ohelp, prim):
+    """A real concurrent run (std::thread + std::atomic) lands on the closed form."""
+    for nw, ncu, L, mode in [(8, 4, 3, 1), (4, 4, 2, 0), (16, 8, 1, 1)]:
+        try:
+            want = ohelp.golden(prim, L, nw, 3, 2, 64, ncu, mode)
+        except AssertionError:
+            continue
+        got = np.zeros_like(want)
+        cs = ctypes.c_int64()
+        t = oracle.hso_cpu_run(prim, L, nw, 3, 2, 64, ncu, mode, got.ctypes.data_as(P32),
+                               got.size, ctypes.byref(cs))
+        assert t >= 0
+        assert np.array_equal(got, want), (prim, nw, ncu, L, mode)
+        w, r, e = ohelp.counts(prim, nw, 3, 2, ncu)
+        if e == 0:
+            assert cs.value == w, (prim, cs.value, w)
+
+
+def test_expected_counts(ohelp):
+    assert ohelp.counts(7, 32, 100, 10, 256) == (32000, 0, 0)  # C1
+    assert ohelp.counts(0, 1024, 1000, 10, 256) == (0, 0, 10000)  # C2
+    w, r, _ = ohelp.counts(8, 512, 5, 10, 256)
+    assert w == 256 * 50 and r == 256 * 50
+
+
+# ---- reference UTS tree (cuda/uts/uvm/uts.cu:290-339) ----------------------
+
+def test_uts_reference_tree_invariants(oracle):
+    n = 8193  # `./uts 1 8192 90 32 128` -> num_nodes = N | 1
+    val = np.zeros(n, np.uint32)
+    ca = np.zeros(n, np.int32)
+    cb = np.zeros(n, np.int32)
+    lg, ls = ctypes.c_uint32(), ctypes.c_uint32()
+    oracle.hso_uts_ref_tree(n, 90, val.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                            ca.ctypes.data_as(P32), cb.ctypes.data_as(P32),
+                            ctypes.byref(lg), ctypes.byref(ls))
+    internal = ca != 0
+    assert np.array_equal(internal, cb != 0)
+    assert np.all(cb[internal] == ca[internal] + 1)
+    assert lg.value == (n + 1) // 2 == int((~internal).sum())
+    assert ls.value == int(val[~internal].sum())
+    # every node except the root is the child of exactly one node
+    kids = np.concatenate([ca[internal], cb[internal]])
+    assert sorted(kids.tolist()) == list(range(1, n))
+    # deterministic (glibc srand(200))
+    val2 = np.zeros(n, np.uint32)
+    oracle.hso_uts_ref_tree(n, 90, val2.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                            ca.ctypes.data_as(P32), cb.ctypes.data_as(P32),
+                            ctypes.byref(lg), ctypes.byref(ls))
+    assert np.array_equal(val, val2)

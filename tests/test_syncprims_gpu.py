@@ -1,0 +1,94 @@
+"""GPU parity of the syncPrims hot path (hs_syncprim_run through the C ABI)
+against the oracle: final storage bit-exact with the closed form (and with
+the literal restatement of the reference golden at small sizes), CS entry /
+episode counts exact, lock and barrier words back in their idle state."""
+import numpy as np
+import pytest
+
+import heterosync_amd as hs
+
+pytestmark = pytest.mark.gpu
+
+MUTEX = [4, 5, 6, 7]
+UMUTEX = [20, 21, 22, 23]
+SEM = list(range(8, 16))
+USEM = list(range(24, 32))
+BARRIER = [0, 1, 2, 3]
+
+
+def check(dev, ohelp, prim, L, nw, iters, reps=10, wg=64, mode=0, literal=False, **kw):
+    ncu = dev.info.num_cu
+    s, r = dev.syncprim(prim, L, nw, iters, wg_size=wg, reps=reps, mode=mode,
+                        timeout_s=10.0, **kw)
+    want = ohelp.golden(prim, L, nw, iters, reps, wg, ncu, mode, literal=literal)
+    assert np.array_equal(s, want), (prim, L, nw, iters, mode, int((s != want).sum()))
+    w, rd, ep = ohelp.counts(prim, nw, iters, reps, ncu)
+    assert (r.cs_entries, r.rd_entries, r.episodes) == (w, rd, ep)
+    assert r.state_ok == 1
+    return r
+
+
+@pytest.mark.parametrize("prim", MUTEX)
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mutex(dev, ohelp, prim, mode):
+    check(dev, ohelp, prim, 10, 32, 20, mode=mode)
+    check(dev, ohelp, prim, 3, 4, 5, mode=mode, literal=True)
+
+
+@pytest.mark.parametrize("prim", UMUTEX)
+def test_uniq_mutex(dev, ohelp, prim):
+    ncu = dev.info.num_cu
+    check(dev, ohelp, prim, 4, 2 * ncu, 10, mode=1)
+    check(dev, ohelp, prim, 4, ncu + 3, 5, mode=1)   # uneven WGs per lock
+    check(dev, ohelp, prim, 2, 8, 5, mode=0, literal=True)
+
+
+@pytest.mark.parametrize("prim", SEM)
+def test_semaphore(dev, ohelp, prim):
+    ncu = dev.info.num_cu
+    check(dev, ohelp, prim, 4, 16, 10, mode=1)
+    check(dev, ohelp, prim, 4, 2 * ncu, 2, reps=2, mode=1)  # 1 writer + readers per CU
+
+
+@pytest.mark.parametrize("prim", USEM)
+def test_uniq_semaphore(dev, ohelp, prim):
+    ncu = dev.info.num_cu
+    check(dev, ohelp, prim, 4, 2 * ncu, 5, mode=1)
+    check(dev, ohelp, prim, 2, 4 * ncu, 3, reps=3, mode=0)
+
+
+@pytest.mark.parametrize("prim", BARRIER)
+@pytest.mark.parametrize("mode", [0, 1])
+def test_barrier_small(dev, ohelp, prim, mode):
+    ncu = dev.info.num_cu
+    check(dev, ohelp, prim, 4, 64, 20, mode=mode)
+    check(dev, ohelp, prim, 4, 4 * ncu, 20, mode=mode)
+    check(dev, ohelp, prim, 2, 2 * ncu, 3, reps=2, mode=mode, wg=128)
+
+
+@pytest.mark.parametrize("prim", BARRIER)
+def test_barrier_full_config(dev, ohelp, prim):
+    """1024 WGs x 256 threads, L=10, 1000 iterations x 10 launches."""
+    ncu = dev.info.num_cu
+    r = check(dev, ohelp, prim, 10, 4 * ncu, 1000, wg=256, mode=1)
+    assert r.episodes == 10000
+
+
+def test_fa_mutex_c1(dev, ohelp):
+    """faMutex 32 WG x 64, numLdSt=10, 100 iterations: 32000 acquires."""
+    for mode in (0, 1):
+        r = check(dev, ohelp, 7, 10, 32, 100, mode=mode)
+        assert r.lock_words[0] == r.lock_words[4] == 32000  # ticket == turn
+
+
+def test_error_paths(dev):
+    ncu = dev.info.num_cu
+    with pytest.raises(hs.HSError) as e:
+        dev.syncprim(16, 10, 8, 1)           # priority semaphore: not built
+    assert e.value.code == -7
+    with pytest.raises(hs.HSError) as e:
+        dev.syncprim(0, 10, ncu + 1, 1)      # barrier grid not a multiple of NUM_CU
+    assert e.value.code == -1
+    with pytest.raises(hs.HSError) as e:
+        dev.syncprim(0, 10, 8 * ncu, 1, wg_size=1024)  # cannot all be co-resident
+    assert e.value.code == -4
