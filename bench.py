@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Benchmark driver: barrier episodes/s on the full-co-residency grid barrier
+workload (1024 workgroups = 4 per CU x 256 CUs, 256 threads, numLdSt=10,
+numCSIters=1000, 10 launches per step), through the C ABI.
+
+One step = one hs_syncprim_run call: 10 launches x 1000 episodes, each launch
+followed by a device synchronize (the reference's plain ROI,
+hip/syncPrims/uvm/main.hip.cpp:2354-2511).  Storage is copied to HBM before
+the ROI; the timed quantity is the device-event ROI.  Prints ONE JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--prim NAME]
+                  [--no-cpu] [--no-extras]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
+METRIC = ("acquire+release/s & barrier episodes/s at 1 GPU, batched vs plain; "
+          "UTS nodes/s, p99")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--prim", default="atomicTreeBarrUniq")
+    ap.add_argument("--num-ldst", type=int, default=10)
+    ap.add_argument("--iters", type=int, default=1000)
+    ap.add_argument("--wg", type=int, default=256)
+    ap.add_argument("--strict", action="store_true",
+                    help="agent-scope release/acquire in every WG")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--cpu-episodes", type=int, default=1500)
+    return ap.parse_args()
+
+
+def cpu_baseline(prim, L, nw, wg, ncu, episodes):
+    """The oracle's threaded CPU port of the same barrier workload, on a
+    bounded sample (`episodes` episodes instead of 10000)."""
+    from tests.conftest import _load_oracle
+    o = _load_oracle()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n = o.hso_num_storage_locs(prim, nw, ncu, wg, L)
+    s = np.zeros(n, np.int32)
+    secs = o.hso_cpu_barrier_run(prim, L, nw, episodes, 1, wg, ncu, 0, threads,
+                                 s.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n)
+    want = np.zeros(n, np.int32)
+    o.hso_golden_closed(prim, L, nw, episodes, 1, wg, ncu, 0,
+                        want.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n)
+    return {
+        "value": round(episodes / secs, 2) if secs > 0 else None,
+        "unit": "barrier episodes/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{episodes} episodes of the same {nw}-WG x {wg}-lane, numLdSt={L} "
+                   f"workload on {threads} std::threads (each owns {nw // threads} "
+                   f"emulated WGs, one std::atomic barrier per episode); "
+                   f"result matches the closed form: {bool(np.array_equal(s, want))}"),
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {a.gpus} requested); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import heterosync_amd as hs
+    from tests.conftest import OracleHelpers, _load_oracle
+
+    oh = OracleHelpers(_load_oracle())
+    dev = hs.Device(0)
+    ncu = dev.info.num_cu
+    prim = hs.prim_id(a.prim)
+    nw = 4 * ncu  # full co-residency: 4 WGs per CU
+    reps = 10
+    scope = 1 if a.strict else 0
+
+    def step():
+        s, r = dev.syncprim(prim, a.num_ldst, nw, a.iters, wg_size=a.wg, reps=reps, mode=0,
+                            timeout_s=30.0, sync_scope=scope)
+        return s, r
+
+    for _ in range(a.warmup):
+        step()
+    roi_ms = kern_ms = 0.0
+    ok = True
+    t0 = time.perf_counter()
+    last = None
+    for _ in range(a.steps):
+        s, r = step()
+        roi_ms += r.roi_ms
+        kern_ms += r.kernel_ms
+        ok = ok and r.state_ok == 1 and r.episodes == a.iters * reps
+        last = (s, r)
+    wall = time.perf_counter() - t0
+    want = oh.golden(prim, a.num_ldst, nw, a.iters, reps, a.wg, ncu, 0)
+    ok = ok and bool(np.array_equal(last[0], want))
+
+    episodes = a.iters * reps * a.steps
+    value = episodes / (roi_ms / 1e3)
+    # roofline: algorithmic bytes per launch = episodes/launch x WGs x lanes x
+    # numLdSt x (4 B load + 4 B store), over the average launch duration
+    bytes_per_episode = nw * a.wg * a.num_ldst * 8
+    avg_launch_ms = kern_ms / (reps * a.steps)
+    achieved = bytes_per_episode * a.iters / (avg_launch_ms / 1e3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "barrier episodes/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(roi_ms / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (storage[i] = i, the reference's initialisation)",
+        "config": {
+            "workload": f"{a.prim}: {nw} WGs x {a.wg} threads, numLdSt={a.num_ldst}, "
+                        f"numCSIters={a.iters}, {reps} launches per step (plain form)",
+            "prim": a.prim, "num_wgs": nw, "wg_size": a.wg, "num_ldst": a.num_ldst,
+            "num_cs_iters": a.iters, "launches_per_step": reps,
+            "sync_scope": "strict-agent" if scope else "xcd-aware",
+            "parity_ok": ok, "wall_s": round(wall, 3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": f"k_barrier ({a.prim})",
+            "avg_launch_ms": round(avg_launch_ms, 4),
+            "bytes_per_episode": bytes_per_episode,
+        },
+    }
+    if not a.no_extras:
+        extras = {}
+        for name in ("atomicTreeBarrUniq", "atomicTreeBarrUniqLocalExch", "lfTreeBarrUniq",
+                     "lfTreeBarrUniqLocalExch"):
+            for sc in (0, 1):
+                _, r = dev.syncprim(name, a.num_ldst, nw, a.iters, wg_size=a.wg, reps=reps,
+                                    timeout_s=30.0, sync_scope=sc)
+                key = f"{name}_{'strict' if sc else 'xcd'}_episodes_per_s"
+                extras[key] = round(r.episodes / (r.roi_ms / 1e3), 1)
+        _, r = dev.syncprim("faMutex", 10, 32, 100, reps=10, timeout_s=30.0)
+        extras["faMutex_32x64_L10_iters100_acq_rel_per_s"] = round(
+            r.cs_entries / (r.roi_ms / 1e3), 1)
+        out["extras"] = extras
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(prim, a.num_ldst, nw, a.wg, ncu, a.cpu_episodes)
+    dev.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -29,7 +29,8 @@ EXPORTED = [
     "hs_strerror", "hs_open", "hs_close", "hs_device_info", "hs_prim_id",
     "hs_prim_name", "hs_prim_supported", "hs_storage_words", "hs_syncprim_run",
     "hs_seqlock_run", "hs_flags_run", "hs_refcounter_run", "hs_splitcounter_run",
-    "hs_uts_ref_run", "hs_uts_geo_run", "hs_batch_run", "hs_uts_serve",
+    "hs_uts_ref_run", "hs_uts_geo_run", "hs_batch_run", "hs_batch_autotune",
+    "hs_uts_serve",
 ]
 
 
@@ -45,6 +46,7 @@ class SyncPrimCfg(ctypes.Structure):
         ("num_cs_iters", ctypes.c_int), ("wg_size", ctypes.c_int), ("reps", ctypes.c_int),
         ("mode", ctypes.c_int), ("backoff_min", ctypes.c_int), ("backoff_max", ctypes.c_int),
         ("poll_sleep", ctypes.c_int), ("timeout_s", ctypes.c_double),
+        ("sync_scope", ctypes.c_int),
     ]
 
 
@@ -53,7 +55,7 @@ class SyncPrimResult(ctypes.Structure):
         ("roi_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double),
         ("cs_entries", ctypes.c_int64), ("rd_entries", ctypes.c_int64),
         ("episodes", ctypes.c_int64), ("state_ok", ctypes.c_int),
-        ("lock_words", ctypes.c_uint32 * 8),
+        ("lock_words", ctypes.c_uint32 * 8), ("sync_scope_used", ctypes.c_int),
     ]
 
 
@@ -151,6 +153,8 @@ def _lib():
         lib.hs_batch_run.argtypes = [ctypes.c_void_p, P(BatchInstance), ctypes.c_int,
                                      P(BatchCfg), P(P(ctypes.c_int32)), P(ctypes.c_int64),
                                      P(BatchResult)]
+        lib.hs_batch_autotune.argtypes = [ctypes.c_void_p, P(BatchInstance), ctypes.c_int,
+                                          P(BatchCfg), P(ctypes.c_double)]
         lib.hs_uts_serve.argtypes = [ctypes.c_void_p, P(UtsRequest), ctypes.c_int,
                                      P(ServeCfg), P(UtsResponse), P(ServeSummary)]
         _LIB = lib
@@ -217,7 +221,8 @@ class Device:
 
     # ---- syncPrims -------------------------------------------------------
     def syncprim(self, prim, num_ldst, num_wgs, num_cs_iters, wg_size=64, reps=10,
-                 mode=0, backoff=(0, 0), poll_sleep=0, timeout_s=10.0, storage=None):
+                 mode=0, backoff=(0, 0), poll_sleep=0, timeout_s=10.0, storage=None,
+                 sync_scope=0):
         """Runs one microbenchmark; returns (storage, SyncPrimResult)."""
         pid = prim if isinstance(prim, int) else prim_id(prim)
         n = storage_words(pid, num_wgs, self.info.num_cu, wg_size, num_ldst)
@@ -226,7 +231,7 @@ class Device:
         if storage is None:
             storage = np.arange(n, dtype=np.int32)
         cfg = SyncPrimCfg(pid, num_ldst, num_wgs, num_cs_iters, wg_size, reps, mode,
-                          backoff[0], backoff[1], poll_sleep, timeout_s)
+                          backoff[0], backoff[1], poll_sleep, timeout_s, sync_scope)
         res = SyncPrimResult()
         _check(_lib().hs_syncprim_run(self._h, ctypes.byref(cfg), _ptr(storage, ctypes.c_int32),
                                       n, ctypes.byref(res)), f"hs_syncprim_run({pid})")
@@ -310,6 +315,16 @@ class Device:
         return list(resp), summ
 
     # ---- batched ---------------------------------------------------------
+    def batch_autotune(self, instances, reps=10, use_graph=1, timeout_s=20.0):
+        """Returns the tuned (persistent_wgs, poll_sleep, backoff_max, best_ms)."""
+        n = len(instances)
+        inst = (BatchInstance * n)(*[BatchInstance(*i) for i in instances])
+        cfg = BatchCfg(64, reps, 0, 0, 0, 0, use_graph, 0, timeout_s)
+        best = ctypes.c_double()
+        _check(_lib().hs_batch_autotune(self._h, inst, n, ctypes.byref(cfg), ctypes.byref(best)),
+               "hs_batch_autotune")
+        return cfg.persistent_wgs, cfg.poll_sleep, cfg.backoff_max, best.value
+
     def batch(self, instances, storages, wg_size=64, reps=10, mode=0, backoff=(0, 0),
               poll_sleep=0, use_graph=1, persistent_wgs=0, timeout_s=20.0):
         n = len(instances)

@@ -38,7 +38,8 @@ void usage() {
                "\t<numWGs>: the # of WGs to execute (want to be divisible by the number of CUs).\n"
                "\t<numCSIters>: number of iterations of the critical section.\n"
                "options: --wg=N (threads/WG, default 64) --mode=copy|inc --reps=N (default 10)\n"
-               "         --backoff=MIN:MAX --poll=N --timeout=SECONDS --json\n");
+               "         --backoff=MIN:MAX --poll=N --timeout=SECONDS --json\n"
+               "         --strict (agent-scope release/acquire in every WG)\n");
 }
 
 const char *banner(int p) {
@@ -98,7 +99,7 @@ int main(int argc, char **argv) {
     usage();
     return -1;
   }
-  int wg = 64, reps = 10, mode = 0, bmin = 0, bmax = 0, poll = 0;
+  int wg = 64, reps = 10, mode = 0, bmin = 0, bmax = 0, poll = 0, strict = 0;
   double timeout = 0;
   bool json = false;
   for (int i = 5; i < argc; ++i) {
@@ -111,6 +112,7 @@ int main(int argc, char **argv) {
     else if (!std::strncmp(a, "--poll=", 7)) poll = std::atoi(a + 7);
     else if (!std::strncmp(a, "--timeout=", 10)) timeout = std::atof(a + 10);
     else if (!std::strcmp(a, "--json")) json = true;
+    else if (!std::strcmp(a, "--strict")) strict = 1;
     else {
       usage();
       return -1;
@@ -187,6 +189,7 @@ int main(int argc, char **argv) {
   cfg.backoff_max = bmax;
   cfg.poll_sleep = poll;
   cfg.timeout_s = timeout;
+  cfg.sync_scope = strict;
   hs_syncprim_result res;
   rc = hs_syncprim_run(ctx, &cfg, storage.data(), n, &res);
   if (rc) {
@@ -229,10 +232,11 @@ int main(int argc, char **argv) {
     const double acq = (double)(res.cs_entries + res.rd_entries);
     std::printf("{\"prim\": \"%s\", \"roi_ms\": %.4f, \"kernel_ms\": %.4f, "
                 "\"cs_entries\": %lld, \"rd_entries\": %lld, \"episodes\": %lld, "
-                "\"acquire_release_per_s\": %.1f, \"episodes_per_s\": %.1f}\n",
+                "\"acquire_release_per_s\": %.1f, \"episodes_per_s\": %.1f, \"sync_scope\": %d}\n",
                 prim_str, res.roi_ms, res.kernel_ms, (long long)res.cs_entries,
                 (long long)res.rd_entries, (long long)res.episodes,
-                secs > 0 ? acq / secs : 0.0, secs > 0 ? res.episodes / secs : 0.0);
+                secs > 0 ? acq / secs : 0.0, secs > 0 ? res.episodes / secs : 0.0,
+                res.sync_scope_used);
   }
   hs_close(ctx);
   return 0;

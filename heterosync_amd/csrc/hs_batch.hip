@@ -1,6 +1,394 @@
-// hs_batch.hip -- batched persistent syncPrims (in progress).
+// hs_batch.hip -- batched persistent form: many independent mutex /
+// semaphore instances (uneven WG counts, iteration counts and numLdSt) run in
+// ONE persistent launch per repeat, the repeats captured in a hipGraph.
+//
+// Each instance keeps the plain form's semantics exactly (the kernels of
+// hip/syncPrims/uvm/main.hip.cpp:645-930, 1147-1214): numWGs "virtual" WGs
+// of 64 lanes, each doing numCSIters {acquire; critical-section walk;
+// release}.  A fixed grid of co-resident wave64 workers multiplexes the
+// virtual WGs (static round-robin assignment).  A worker never blocks on one
+// virtual WG: it makes one non-blocking attempt (TTAS, ticket check, queue
+// slot check or semaphore CAS) and moves on, so a spinning reader can never
+// starve the writer it waits for, whatever the residency.  Backoff becomes
+// "skip this virtual WG for N rounds".
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "hs_host.h"
-extern "C" {
-int hs_batch_run(hs_ctx *, const hs_batch_instance *, int, const hs_batch_cfg *,
-                 int32_t *const *, int64_t *, hs_batch_result *) { return HS_EUNSUPPORTED; }
+#include "hs_syncprims_dev.h"
+
+using namespace hs;
+
+namespace {
+
+constexpr int kMaxVpw = 64;  // virtual WGs per worker
+
+struct BInst {
+  int32_t *storage;
+  uint32_t *lk;
+  uint32_t *ring;
+  int ring_stride, prim, nw, iters, L, nab, per, vbase;
+};
+
+struct BArgs {
+  const BInst *inst;
+  int ninst, nv, vpw;
+  uint32_t *ctl;
+  uint64_t ticks;
+  Backoff bo;
+};
+
+enum { kIdle = 0, kWaiting = 1, kDone = 2 };
+
+__device__ __forceinline__ int find_inst(const BInst *in, int n, int v) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (in[mid].vbase <= v) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(64) k_batch(BArgs a) {
+  __shared__ int s_inst[kMaxVpw], s_b[kMaxVpw], s_iter[kMaxVpw], s_phase[kMaxVpw];
+  __shared__ uint32_t s_ticket[kMaxVpw], s_skip[kMaxVpw], s_bo[kMaxVpw];
+  __shared__ int s_remaining, s_enter, s_ok;
+  __shared__ int32_t sink[64];
+  const int lane = threadIdx.x, P = gridDim.x, w = blockIdx.x;
+  Guard g;
+  g.init(a.ctl, a.ticks);
+  sink[lane] = 0;
+  if (lane == 0) {
+    int rem = 0;
+    for (int j = 0; j < a.vpw; ++j) {
+      const int v = w + j * P;
+      if (v < a.nv) {
+        const int i = find_inst(a.inst, a.ninst, v);
+        s_inst[j] = i;
+        s_b[j] = v - a.inst[i].vbase;
+        s_phase[j] = a.inst[i].iters > 0 ? kIdle : kDone;
+        rem += a.inst[i].iters > 0;
+      } else {
+        s_phase[j] = kDone;
+      }
+      s_iter[j] = 0;
+      s_skip[j] = 0;
+      s_bo[j] = a.bo.min_q;
+    }
+    s_remaining = rem;
+    s_ok = !g.aborted();
+  }
+  __syncthreads();
+  while (s_remaining > 0 && s_ok) {
+    int progress = 0;
+    for (int j = 0; j < a.vpw; ++j) {
+      if (s_phase[j] == kDone) continue;  // uniform (LDS, read after a barrier)
+      const BInst &I = a.inst[s_inst[j]];
+      const int p = I.prim;
+      const bool sem = p >= 8;
+      const int b = s_b[j];
+      const int grp = b % I.nab, pw = b / I.nab;
+      const bool writer = !sem || pw == grp % I.per;  // main.hip.cpp:883
+      if (lane == 0) {
+        int enter = 0;
+        if (s_skip[j] > 0) {
+          --s_skip[j];
+        } else if (!sem) {
+          uint32_t *lk = I.lk;
+          if (p == 4 || p == 5) {  // spin / EBO: one test-and-test-and-set
+            enter = ld_rlx(lk + kLkHead) == 0 && cas_acq(lk + kLkHead, 0, 1) == 0;
+            if (!enter && p == 5) {
+              s_skip[j] = s_bo[j] >> 2;
+              s_bo[j] = ((s_bo[j] << 1) + 1) & a.bo.max_q;
+            }
+          } else if (p == 7) {  // ticket: take one, then check the turn
+            if (s_phase[j] == kIdle) {
+              s_ticket[j] = add_rlx(lk + kLkHead, 1);
+              s_phase[j] = kWaiting;
+            }
+            enter = ld_rlx(lk + kLkTail) == s_ticket[j];
+            if (enter) fence_acq();
+          } else {  // p == 6, queue lock: take a slot, then check it
+            if (s_phase[j] == kIdle) {
+              s_ticket[j] = add_rlx(lk + kLkTail, 1);
+              s_phase[j] = kWaiting;
+            }
+            enter = ld_rlx(I.ring + (long)s_ticket[j] * I.ring_stride) == 1u;
+            if (enter) fence_acq();
+          }
+        } else {
+          const int r = sem_try_wait(I.lk, writer, g);
+          if (r < 0) s_ok = 0;
+          enter = r > 0;
+          if (!enter && p >= 12) {  // EBO semaphore
+            s_skip[j] = s_bo[j] >> 2;
+            s_bo[j] = writer ? ((s_bo[j] << 1) + 1) & a.bo.max_q : s_bo[j] + 5;
+            if (s_bo[j] > a.bo.max_q) s_bo[j] = a.bo.max_q;
+          }
+        }
+        s_enter = enter;
+      }
+      __syncthreads();
+      if (!s_enter) continue;
+      // ---- critical section of virtual WG b of instance I --------------
+      if (!sem) {
+        wg_access<0>(I.storage, 0, I.L);
+      } else if (writer) {
+        for (int k = 0; k < I.per; ++k) wg_access<0>(I.storage, (long)k * 64, I.L);
+      } else {
+        wg_read(I.storage, (long)pw * 64, I.L, sink);
+      }
+      __syncthreads();
+      if (lane == 0) {
+        uint32_t *lk = I.lk;
+        if (!sem) {
+          st_rlx(lk + kLkCs, ld_rlx(lk + kLkCs) + 1);
+          if (p == 4 || p == 5) mutex_unlock<kSpin>(lk, nullptr, 0, 0);
+          else if (p == 7) mutex_unlock<kFA>(lk, nullptr, 0, s_ticket[j]);
+          else mutex_unlock<kSleep>(lk, I.ring, I.ring_stride, s_ticket[j]);
+        } else {
+          if (writer) st_rlx(lk + kLkCs, ld_rlx(lk + kLkCs) + 1);
+          else add_rlx(lk + kLkRd, 1);
+          sem_post(lk, writer, a.bo, g);
+        }
+        s_bo[j] = a.bo.min_q;
+        s_phase[j] = ++s_iter[j] == I.iters ? kDone : kIdle;
+        if (s_phase[j] == kDone) --s_remaining;
+      }
+      progress = 1;
+      __syncthreads();
+    }
+    if (lane == 0 && !progress) {
+      if (!g.ok()) s_ok = 0;
+      sleep_quanta(a.bo.poll_q);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_batch_init(uint32_t *lk, uint32_t sem_max, uint32_t *ring, long ring_len,
+                             int stride) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && sem_max) {
+    lk[kLkHead] = sem_init_word(sem_max);
+    lk[kLkMax] = sem_max;
+  }
+  if (ring)
+    for (long k = i; k < ring_len; k += (long)gridDim.x * blockDim.x)
+      ring[k * stride] = k == 0 ? 1u : 0xffffffffu;
+}
+
+uint32_t sem_max_of(int p) {
+  static const uint32_t kN[4] = {1, 2, 10, 120};
+  return p >= 8 && p <= 15 ? kN[(p - 8) % 4] + 1 : 0;
+}
+
+struct Batch {
+  DevArena ar;
+  std::vector<BInst> h;
+  std::vector<long> words;
+  BInst *d_inst = nullptr;
+  uint32_t *d_ctl = nullptr;
+  int nv = 0;
+};
+
+int batch_prepare(hs_ctx *c, const hs_batch_instance *in, int n, int reps, Batch &B,
+                  int32_t *const *storages) {
+  const int ncu = c->num_cu;
+  B.h.resize(n);
+  B.words.resize(n);
+  int rc;
+  for (int i = 0; i < n; ++i) {
+    const int p = in[i].prim, nw = in[i].num_wgs;
+    if (p < 4 || p > 15 || nw < 1 || nw > c->max_wgs || in[i].num_cs_iters < 0 ||
+        in[i].num_ldst < 0)
+      return HS_EINVAL;
+    BInst &I = B.h[i];
+    std::memset(&I, 0, sizeof I);
+    I.prim = p;
+    I.nw = nw;
+    I.iters = in[i].num_cs_iters;
+    I.L = in[i].num_ldst;
+    I.nab = std::min(nw, ncu);
+    I.per = (nw + I.nab - 1) / I.nab;
+    I.vbase = B.nv;
+    B.nv += nw;
+    B.words[i] = hs_storage_words(p, nw, ncu, 64, I.L);
+    if ((rc = B.ar.alloc(&I.storage, B.words[i]))) return rc;
+    if ((rc = B.ar.alloc(&I.lk, kLockWords))) return rc;
+    HS_HIP(hipMemsetAsync(I.lk, 0, kLockWords * 4, c->stream));
+    long ring_len = 0;
+    if (p == 6) {
+      ring_len = (long)nw * I.iters * reps + 1;
+      I.ring_stride = ring_len <= (1L << 18) ? HS_LINE_WORDS : 1;
+      if ((rc = B.ar.alloc(&I.ring, (size_t)ring_len * I.ring_stride))) return rc;
+    }
+    const uint32_t sm = sem_max_of(p);
+    if (sm || I.ring) {
+      const int blocks = (int)std::min<long>((ring_len + 255) / 256 + 1, 1024);
+      k_batch_init<<<blocks, 256, 0, c->stream>>>(I.lk, sm, I.ring, ring_len, I.ring_stride);
+      HS_HIP(hipGetLastError());
+    }
+    if (storages) {
+      HS_HIP(hipMemcpyAsync(I.storage, storages[i], B.words[i] * 4, hipMemcpyHostToDevice,
+                            c->stream));
+    } else {
+      std::vector<int32_t> init(B.words[i]);
+      for (long k = 0; k < B.words[i]; ++k) init[k] = (int32_t)k;
+      HS_HIP(hipMemcpyAsync(I.storage, init.data(), B.words[i] * 4, hipMemcpyHostToDevice,
+                            c->stream));
+      HS_HIP(hipStreamSynchronize(c->stream));
+    }
+  }
+  if ((rc = B.ar.alloc(&B.d_inst, n))) return rc;
+  if ((rc = B.ar.alloc(&B.d_ctl, 64))) return rc;
+  HS_HIP(hipMemcpyAsync(B.d_inst, B.h.data(), sizeof(BInst) * n, hipMemcpyHostToDevice,
+                        c->stream));
+  HS_HIP(hipMemsetAsync(B.d_ctl, 0, 256, c->stream));
+  HS_HIP(hipStreamSynchronize(c->stream));
+  return HS_OK;
+}
+
+// Runs `reps` persistent launches (optionally as one captured graph) and
+// returns the ROI in ms.
+int batch_launch(hs_ctx *c, Batch &B, int n, const hs_batch_cfg *cfg, int reps, int workers,
+                 double *roi_ms, double *kern_ms) {
+  BArgs a;
+  a.inst = B.d_inst;
+  a.ninst = n;
+  a.nv = B.nv;
+  const int P = std::max(1, std::min(workers, B.nv));
+  a.vpw = (B.nv + P - 1) / P;
+  if (a.vpw > kMaxVpw) return HS_EINVAL;
+  a.ctl = B.d_ctl;
+  a.ticks = timeout_ticks(c, cfg->timeout_s);
+  a.bo.min_q = cfg->backoff_min > 0 ? cfg->backoff_min : 1;
+  a.bo.max_q = cfg->backoff_max > 0 ? cfg->backoff_max : 1023;
+  a.bo.poll_q = cfg->poll_sleep < 0 ? 0 : (cfg->poll_sleep > 0 ? cfg->poll_sleep : 1);
+  hipEvent_t e0, e1;
+  HS_HIP(hipEventCreate(&e0));
+  HS_HIP(hipEventCreate(&e1));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (cfg->use_graph) {
+    HS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_batch, dim3(P), dim3(64), 0, c->stream, a);
+    HS_HIP(hipStreamEndCapture(c->stream, &graph));
+    HS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  HS_HIP(hipDeviceSynchronize());
+  HS_HIP(hipEventRecord(e0, c->stream));
+  if (cfg->use_graph) {
+    HS_HIP(hipGraphLaunch(exec, c->stream));
+  } else {
+    for (int r = 0; r < reps; ++r) {
+      hipLaunchKernelGGL(k_batch, dim3(P), dim3(64), 0, c->stream, a);
+      HS_HIP(hipGetLastError());
+    }
+  }
+  HS_HIP(hipEventRecord(e1, c->stream));
+  HS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  HS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *roi_ms = ms;
+  if (kern_ms) *kern_ms = ms;
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return HS_OK;
+}
+
+int default_workers(const hs_ctx *c, const hs_batch_cfg *cfg, int nv) {
+  const int per_cu = cfg->persistent_wgs > 0 ? 0 : 8;
+  int P = cfg->persistent_wgs > 0 ? cfg->persistent_wgs : per_cu * c->num_cu;
+  while ((nv + P - 1) / P > kMaxVpw) P *= 2;
+  return P;
+}
+
+}  // namespace
+
+extern "C" int hs_batch_run(hs_ctx *c, const hs_batch_instance *in, int n,
+                            const hs_batch_cfg *cfg, int32_t *const *storages, int64_t *entries,
+                            hs_batch_result *res) {
+  if (!c || !in || n < 1 || !cfg || !storages || !entries || !res) return HS_EINVAL;
+  if (cfg->wg_size != 0 && cfg->wg_size != 64) return HS_EINVAL;  // wave64 virtual WGs
+  if (cfg->mode != 0) return HS_EINVAL;
+  std::memset(res, 0, sizeof *res);
+  const int reps = cfg->reps > 0 ? cfg->reps : 10;
+  HS_HIP(hipSetDevice(c->device));
+  Batch B;
+  int rc = batch_prepare(c, in, n, reps, B, storages);
+  if (rc) return rc;
+  const int P = default_workers(c, cfg, B.nv);
+  int fit = 0;
+  if ((rc = coresident_wgs(c, (const void *)k_batch, 64, &fit))) return rc;
+  if (fit < std::min(P, B.nv)) return HS_EOCCUPANCY;
+  double roi = 0, kern = 0;
+  if ((rc = batch_launch(c, B, n, cfg, reps, P, &roi, &kern))) return rc;
+  res->roi_ms = roi;
+  res->kernel_ms = kern;
+  uint32_t ctl[64];
+  HS_HIP(hipMemcpy(ctl, B.d_ctl, sizeof ctl, hipMemcpyDeviceToHost));
+  bool ok = true;
+  for (int i = 0; i < n; ++i) {
+    const BInst &I = B.h[i];
+    HS_HIP(hipMemcpy(storages[i], I.storage, B.words[i] * 4, hipMemcpyDeviceToHost));
+    uint32_t lk[kLockWords];
+    HS_HIP(hipMemcpy(lk, I.lk, sizeof lk, hipMemcpyDeviceToHost));
+    entries[i] = (int64_t)lk[kLkCs] + lk[kLkRd];
+    res->acquires += entries[i];
+    const uint64_t acq = (uint64_t)I.nw * I.iters * reps;
+    const int p = I.prim;
+    if (p == 4 || p == 5) ok = ok && lk[kLkHead] == 0;
+    if (p == 7) ok = ok && lk[kLkHead] == (uint32_t)acq && lk[kLkTail] == (uint32_t)acq;
+    if (p == 6) ok = ok && lk[kLkTail] == (uint32_t)acq;
+    if (p >= 8) ok = ok && lk[kLkHead] == sem_max_of(p);
+    ok = ok && (uint64_t)entries[i] == acq;
+  }
+  res->all_ok = ok && ctl[0] == 0;
+  return ctl[0] ? HS_ETIMEOUT : HS_OK;
+}
+
+extern "C" int hs_batch_autotune(hs_ctx *c, const hs_batch_instance *in, int n,
+                                 hs_batch_cfg *cfg, double *best_ms) {
+  if (!c || !in || n < 1 || !cfg) return HS_EINVAL;
+  HS_HIP(hipSetDevice(c->device));
+  // calibration copy: iterations capped so each trial takes milliseconds
+  std::vector<hs_batch_instance> cal(in, in + n);
+  for (auto &x : cal) x.num_cs_iters = std::min(x.num_cs_iters, 20);
+  static const int kWorkersPerCu[] = {2, 4, 8, 16};
+  static const int kPoll[] = {0, 1, 4};
+  static const int kBackoffMax[] = {63, 255, 1023};
+  double best = 1e30;
+  hs_batch_cfg best_cfg = *cfg;
+  for (int wpc : kWorkersPerCu)
+    for (int poll : kPoll)
+      for (int bmax : kBackoffMax) {
+        hs_batch_cfg t = *cfg;
+        t.persistent_wgs = wpc * c->num_cu;
+        t.poll_sleep = poll == 0 ? -1 : poll;
+        t.backoff_max = bmax;
+        t.reps = 1;
+        Batch B;
+        int rc = batch_prepare(c, cal.data(), n, 1, B, nullptr);
+        if (rc) return rc;
+        const int P = default_workers(c, &t, B.nv);
+        double roi = 0;
+        if ((rc = batch_launch(c, B, n, &t, 1, P, &roi, nullptr))) return rc;
+        uint32_t ab = 0;
+        HS_HIP(hipMemcpy(&ab, B.d_ctl, 4, hipMemcpyDeviceToHost));
+        if (ab) return HS_ETIMEOUT;
+        if (roi < best) {
+          best = roi;
+          best_cfg = t;
+        }
+      }
+  best_cfg.reps = cfg->reps;
+  *cfg = best_cfg;
+  if (best_ms) *best_ms = best;
+  return HS_OK;
 }

@@ -32,10 +32,13 @@ struct SPArgs {
   int ring_stride;
   TreeBar tb;
   FlagBar fb;
-  int L, iters, num_cu, nab, per_cu;
+  XcdTree xt;
+  XcdFlags xf;
+  int L, iters, num_cu, nab, per_cu, nxcd;
   uint32_t e_global0, e_local0;
   Backoff bo;
   uint64_t ticks;
+  uint32_t *xcd_map;  // per launch: physical XCD of each logical XCD (b % nxcd)
 };
 
 template <int KIND, bool UNIQ, int MODE>
@@ -110,7 +113,7 @@ __global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
   }
 }
 
-template <bool FLAG, bool LX, int MODE>
+template <bool FLAG, bool LX, int MODE, bool XCD>
 __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
   __shared__ int s_ok, s_flag;
   Guard g;
@@ -119,31 +122,54 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
   const int grp = b % nab, q = b / nab;  // cuID, perCU_WGID (HM:122-132)
   int cur = b;
   uint32_t eg = a.e_global0, el = a.e_local0;
-  if (is_master()) s_ok = !g.aborted();
+  if (is_master()) {
+    s_ok = !g.aborted();
+    // The XCD-aware protocol needs every WG with the same b % nxcd on the
+    // same XCD (round-robin dispatch; the starting XCD varies per launch).
+    if (XCD) {
+      const uint32_t me = xcc_id();
+      const uint32_t old = cas_rlx(a.xcd_map + b % a.nxcd, 0xffffffffu, me);
+      if (old != 0xffffffffu && old != me) {
+        st_rlx(a.ctl + 32, 1);
+        st_rlx(a.ctl, 1);
+        s_ok = 0;
+      }
+    }
+  }
   __syncthreads();
   if (!s_ok) return;
   for (int i = 0; i < a.iters; ++i) {
-    wg_access<MODE>(a.storage, (long)cur * blockDim.x, a.L);
+    wg_access<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
     if (LX && a.per_cu > 1) {
       bool ok;
       if (FLAG) {
-        ok = flag_tree_barrier(a.fb, grp, q, ++el, 0, true, a.bo.poll_q, g, &s_flag);
+        ok = XCD ? xcd_flag_barrier(a.xf, grp, q, ++el, 0, true, a.bo.poll_q, g, &s_flag)
+                 : flag_tree_barrier(a.fb, grp, q, ++el, 0, true, a.bo.poll_q, g, &s_flag);
       } else {
+        if (XCD) wait_mem();
         __syncthreads();
-        if (is_master()) s_ok = atomic_tree_arrive(a.tb, grp, true, a.bo.poll_q, g);
+        if (is_master())
+          s_ok = XCD ? xcd_tree_arrive(a.xt, grp, true, a.bo.poll_q, g)
+                     : atomic_tree_arrive(a.tb, grp, true, a.bo.poll_q, g);
         __syncthreads();
         ok = s_ok;
       }
       if (!ok) return;
       cur = (cur + nab) % grid;  // HM:355-358
-      wg_access<MODE>(a.storage, (long)cur * blockDim.x, a.L);
+      wg_access<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
     }
     bool ok;
     if (FLAG) {
-      ok = flag_tree_barrier(a.fb, grp, q, ++el, ++eg, false, a.bo.poll_q, g, &s_flag);
+      ++el;
+      ++eg;
+      ok = XCD ? xcd_flag_barrier(a.xf, grp, q, el, eg, false, a.bo.poll_q, g, &s_flag)
+               : flag_tree_barrier(a.fb, grp, q, el, eg, false, a.bo.poll_q, g, &s_flag);
     } else {
+      if (XCD) wait_mem();
       __syncthreads();
-      if (is_master()) s_ok = atomic_tree_arrive(a.tb, grp, false, a.bo.poll_q, g);
+      if (is_master())
+        s_ok = XCD ? xcd_tree_arrive(a.xt, grp, false, a.bo.poll_q, g)
+                   : atomic_tree_arrive(a.tb, grp, false, a.bo.poll_q, g);
       __syncthreads();
       ok = s_ok;
     }
@@ -158,9 +184,8 @@ __global__ void k_init_locks(uint32_t *locks, int nlocks, uint32_t sem_max,
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nlocks && sem_max) {
     uint32_t *lk = locks + i * kLockWords;
-    lk[kLkCount] = sem_max;
+    lk[kLkHead] = sem_init_word(sem_max);  // count = max, writerWaiting = 1 (HSS:15-28)
     lk[kLkMax] = sem_max;
-    lk[kLkWaiting] = 1;  // HSS:23: the writer goes first
   }
   if (ring) {
     const long total = (long)nlocks * ring_len;
@@ -175,13 +200,13 @@ bool is_sem(int p) { return (p >= 8 && p <= 15) || (p >= 24 && p <= 31); }
 
 using KernelFn = void (*)(SPArgs);
 
-template <int MODE>
+template <int MODE, bool XCD>
 KernelFn pick_kernel(int p) {
   switch (p) {
-    case 0: return k_barrier<false, false, MODE>;
-    case 1: return k_barrier<false, true, MODE>;
-    case 2: return k_barrier<true, false, MODE>;
-    case 3: return k_barrier<true, true, MODE>;
+    case 0: return k_barrier<false, false, MODE, XCD>;
+    case 1: return k_barrier<false, true, MODE, XCD>;
+    case 2: return k_barrier<true, false, MODE, XCD>;
+    case 3: return k_barrier<true, true, MODE, XCD>;
     case 4: return k_mutex<kSpin, false, MODE>;
     case 5: return k_mutex<kEBO, false, MODE>;
     case 6: return k_mutex<kSleep, false, MODE>;
@@ -206,11 +231,37 @@ uint32_t sem_count_of(int p) {
   return 0;
 }
 
+// XCDs of the device: gfx950 (MI355X) / gfx942 (MI300X) have 32 CUs per XCD.
+int num_xcds(const hs_ctx *c) {
+  if (std::strncmp(c->arch, "gfx950", 6) && std::strncmp(c->arch, "gfx942", 6)) return 1;
+  return c->num_cu >= 64 && c->num_cu % 32 == 0 ? c->num_cu / 32 : 1;
+}
+
+int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
+             hs_syncprim_result *res, bool xcd_mode, bool *topology_fail);
+
 }  // namespace
 
 extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
                                int32_t *storage, long n, hs_syncprim_result *res) {
   if (!c || !cfg || !storage || !res) return HS_EINVAL;
+  const int p = cfg->prim;
+  const bool xcd = cfg->sync_scope == 0 && p >= 0 && p < 4 && num_xcds(c) > 1;
+  if (!xcd) return run_once(c, cfg, storage, n, res, false, nullptr);
+  std::vector<int32_t> init(storage, storage + (n > 0 ? n : 0));
+  bool topo = false;
+  int rc = run_once(c, cfg, storage, n, res, true, &topo);
+  if (rc == HS_ETIMEOUT && topo) {  // placement was not round-robin: strict form
+    std::copy(init.begin(), init.end(), storage);
+    rc = run_once(c, cfg, storage, n, res, false, nullptr);
+  }
+  return rc;
+}
+
+namespace {
+
+int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
+             hs_syncprim_result *res, bool xcd_mode, bool *topology_fail) {
   std::memset(res, 0, sizeof(*res));
   const int p = cfg->prim;
   if (hs_prim_name(p) == nullptr) return HS_EINVAL;
@@ -229,7 +280,10 @@ extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
   if (is_sem(p) && uniq && nw % nab) return HS_EINVAL;  // each group needs its writer
   if (n != hs_storage_words(p, nw, ncu, wg, L)) return HS_EINVAL;
 
-  KernelFn kfn = cfg->mode ? pick_kernel<1>(p) : pick_kernel<0>(p);
+  KernelFn kfn = xcd_mode ? (cfg->mode ? pick_kernel<1, true>(p) : pick_kernel<0, true>(p))
+                          : (cfg->mode ? pick_kernel<1, false>(p) : pick_kernel<0, false>(p));
+  const int nxcd = xcd_mode ? num_xcds(c) : 1;
+  res->sync_scope_used = xcd_mode ? 0 : 1;
   if (!kfn) return HS_EUNSUPPORTED;
   HS_HIP(hipSetDevice(c->device));
   if (is_barrier(p) || is_sem(p)) {  // forward progress needs co-residency
@@ -260,10 +314,14 @@ extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
   if ((rc = arena.alloc(&d_locks, (size_t)nlocks * kLockWords))) return rc;
   if (sleep_mx && (rc = arena.alloc(&d_ring, (size_t)nlocks * ring_len * ring_stride)))
     return rc;
-  const size_t bar_words = (size_t)kBarWords * (1 + nab) + HS_LINE_WORDS;
+  const size_t bar_words = (size_t)kBarWords * (1 + nxcd + nab) + HS_LINE_WORDS;
   if ((rc = arena.alloc(&d_bar, bar_words))) return rc;
-  const size_t flag_words = 2 * (size_t)nab + 2 * (size_t)nab * per + 4 * HS_LINE_WORDS;
+  const size_t flag_words =
+      2 * (size_t)nxcd + 2 * (size_t)nab + 2 * (size_t)nab * per + 4 * HS_LINE_WORDS;
   if ((rc = arena.alloc(&d_flags, flag_words))) return rc;
+  uint32_t *d_xmap;
+  if ((rc = arena.alloc(&d_xmap, (size_t)reps * nxcd))) return rc;
+  HS_HIP(hipMemsetAsync(d_xmap, 0xff, (size_t)reps * nxcd * 4, c->stream));
   HS_HIP(hipMemsetAsync(d_ctl, 0, 64 * sizeof(uint32_t), c->stream));
   HS_HIP(hipMemsetAsync(d_locks, 0, (size_t)nlocks * kLockWords * 4, c->stream));
   HS_HIP(hipMemsetAsync(d_bar, 0, bar_words * 4, c->stream));
@@ -288,12 +346,17 @@ extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
   a.ring = d_ring;
   a.ring_len = ring_len;
   a.ring_stride = ring_stride;
-  uint32_t *episodes = d_bar + kBarWords * (1 + nab);
+  uint32_t *episodes = d_bar + kBarWords * (1 + nxcd + nab);
+  // strict: global block, nab group blocks; XCD: global, nxcd XCD blocks, nab groups
   a.tb = TreeBar{d_bar, d_bar + kBarWords, episodes, nab, per};
-  a.fb = FlagBar{d_flags + 4 * HS_LINE_WORDS, d_flags + 4 * HS_LINE_WORDS + nab,
-                 d_flags + 4 * HS_LINE_WORDS + 2 * nab,
-                 d_flags + 4 * HS_LINE_WORDS + 2 * nab + (size_t)nab * per, episodes,
+  a.xt = XcdTree{d_bar, d_bar + kBarWords, d_bar + kBarWords * (1 + nxcd), episodes, nab,
+                 per, nxcd};
+  uint32_t *fl = d_flags + 4 * HS_LINE_WORDS;
+  a.fb = FlagBar{fl, fl + nab, fl + 2 * nab, fl + 2 * nab + (size_t)nab * per, episodes,
                  nab, per};
+  a.xf = XcdFlags{fl, fl + nxcd, fl + 2 * nxcd, fl + 2 * nxcd + nab, fl + 2 * nxcd + 2 * nab,
+                  fl + 2 * nxcd + 2 * nab + (size_t)nab * per, episodes, nab, per, nxcd};
+  a.nxcd = nxcd;
   a.L = L;
   a.iters = iters;
   a.num_cu = ncu;
@@ -316,6 +379,7 @@ extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
   HS_HIP(hipEventRecord(ev0, c->stream));
   int launch_rc = HS_OK;
   for (int r = 0; r < reps; ++r) {
+    a.xcd_map = d_xmap + (size_t)r * nxcd;
     a.e_global0 = (uint32_t)r * iters;
     a.e_local0 = (uint32_t)r * iters * local_per_iter;
     HS_HIP(hipEventRecord(kev[2 * r], c->stream));
@@ -343,8 +407,10 @@ extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
 
   // ---- results and idle-state invariants ----------------------------------
   HS_HIP(hipMemcpy(storage, d_storage, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-  uint32_t abort_w = 0;
-  HS_HIP(hipMemcpy(&abort_w, d_ctl, 4, hipMemcpyDeviceToHost));
+  uint32_t ctlw[64];
+  HS_HIP(hipMemcpy(ctlw, d_ctl, sizeof ctlw, hipMemcpyDeviceToHost));
+  const uint32_t abort_w = ctlw[0];
+  if (topology_fail) *topology_fail = ctlw[32] != 0;
   std::vector<uint32_t> locks((size_t)nlocks * kLockWords), bar(bar_words), flags(flag_words);
   HS_HIP(hipMemcpy(locks.data(), d_locks, locks.size() * 4, hipMemcpyDeviceToHost));
   HS_HIP(hipMemcpy(bar.data(), d_bar, bar.size() * 4, hipMemcpyDeviceToHost));
@@ -374,37 +440,51 @@ extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
           ok = false;
         }
       }
-      if (is_sem(p) && (!uniq || l < nab))
-        ok = ok && lk[kLkHead] == 0 && lk[kLkCount] == sem_max && lk[kLkWaiting] == 0;
+      if (is_sem(p) && (!uniq || l < nab))  // count back to max, no writer waiting
+        ok = ok && lk[kLkHead] == sem_max;
     }
     const uint32_t *lk0 = &locks[0];
-    const uint32_t w[8] = {lk0[kLkHead], lk0[kLkCount], lk0[kLkWaiting], lk0[kLkMax],
+    const bool semw = is_sem(p);
+    const uint32_t w[8] = {lk0[kLkHead], semw ? (lk0[kLkHead] & 0xffffu) : 0u,
+                           semw ? (lk0[kLkHead] >> 16) : 0u, lk0[kLkMax],
                            lk0[kLkTail], lk0[kLkCs], lk0[kLkRd], (uint32_t)nlocks};
     std::memcpy(res->lock_words, w, sizeof(w));
   } else {
-    const uint32_t ep = bar[kBarWords * (1 + nab)];
+    const uint32_t ep = bar[kBarWords * (1 + nxcd + nab)];
     res->episodes = ep;
     ok = ok && ep == (uint32_t)total_ep;
+    const uint32_t te = (uint32_t)total_ep, tl = (uint32_t)(total_ep * local_per_iter);
     if (p <= 1) {
-      ok = ok && bar[0] == 0 && bar[32] == (uint32_t)total_ep;
+      const size_t g0 = xcd_mode ? kBarWords * (1 + nxcd) : kBarWords;
+      ok = ok && bar[0] == 0 && bar[32] == te;
+      if (xcd_mode)
+        for (int x = 0; x < std::min(nab, nxcd); ++x)
+          ok = ok && bar[kBarWords * (1 + x)] == 0 && bar[kBarWords * (1 + x) + 32] == te;
       if (per > 1)
         for (int gi = 0; gi < nab; ++gi)
-          ok = ok && bar[kBarWords * (1 + gi)] == 0 &&
-               bar[kBarWords * (1 + gi) + 32] == (uint32_t)(total_ep * local_per_iter);
-      const uint32_t w[8] = {bar[0], bar[32], bar[kBarWords], bar[kBarWords + 32], ep, 0, 0, 0};
+          ok = ok && bar[g0 + kBarWords * gi] == 0 && bar[g0 + kBarWords * gi + 32] == tl;
+      const uint32_t w[8] = {bar[0], bar[32], bar[g0], bar[g0 + 32], ep, 0, 0, 0};
       std::memcpy(res->lock_words, w, sizeof(w));
     } else {
-      const uint32_t *gin = &flags[4 * HS_LINE_WORDS], *gout = gin + nab;
-      const uint32_t *lin = gout + nab, *lout = lin + (size_t)nab * per;
-      for (int gi = 1; gi < nab; ++gi)
-        ok = ok && gin[gi] == (uint32_t)total_ep && gout[gi] == (uint32_t)total_ep;
+      const uint32_t *f = &flags[4 * HS_LINE_WORDS];
+      const uint32_t *lin, *lout;
+      if (xcd_mode) {
+        const uint32_t *gin = f, *gout = f + nxcd, *xin = f + 2 * nxcd, *xout = xin + nab;
+        for (int x = 1; x < std::min(nab, nxcd); ++x) ok = ok && gin[x] == te && gout[x] == te;
+        for (int gi = nxcd; gi < nab; ++gi) ok = ok && xin[gi] == te && xout[gi] == te;
+        lin = xout + nab;
+      } else {
+        const uint32_t *gin = f, *gout = f + nab;
+        for (int gi = 1; gi < nab; ++gi) ok = ok && gin[gi] == te && gout[gi] == te;
+        lin = gout + nab;
+      }
+      lout = lin + (size_t)nab * per;
       if (per > 1)
         for (int gi = 0; gi < nab; ++gi)
           for (int m = 1; m < per; ++m)
-            ok = ok && lin[(size_t)gi * per + m] == (uint32_t)(total_ep * local_per_iter) &&
-                 lout[(size_t)gi * per + m] == (uint32_t)(total_ep * local_per_iter);
-      const uint32_t w[8] = {nab > 1 ? gin[1] : 0, nab > 1 ? gout[1] : 0,
-                             per > 1 ? lin[1] : 0, per > 1 ? lout[1] : 0, ep, 0, 0, 0};
+            ok = ok && lin[(size_t)gi * per + m] == tl && lout[(size_t)gi * per + m] == tl;
+      const uint32_t w[8] = {f[1], f[1 + (xcd_mode ? nxcd : nab)], per > 1 ? lin[1] : 0,
+                             per > 1 ? lout[1] : 0, ep, 0, 0, 0};
       std::memcpy(res->lock_words, w, sizeof(w));
     }
   }
@@ -412,3 +492,5 @@ extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
   if (abort_w) return HS_ETIMEOUT;
   return HS_OK;
 }
+
+}  // namespace

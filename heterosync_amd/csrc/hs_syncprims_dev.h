@@ -16,15 +16,14 @@ namespace hs {
 constexpr int kWordsPerLine = 16;  // oracle layout constant (HM:12)
 
 // Lock block: 128 words (512 B) per lock instance.
-//   word  0 : lock / head / ticket        (line 0, polled by spin/EBO)
-//   word  1 : semaphore count              (line 0, only under the lock)
-//   word  2 : semaphore writerWaiting      (line 0, only under the lock)
+//   word  0 : lock / head / ticket        (line 0, polled by spin/EBO);
+//             semaphore state: count | writerWaiting << 16
 //   word  3 : semaphore max count
 //   word 32 : tail / turn                 (line 1, polled by FA)
 //   word 64 : writer / CS entry counter   (line 2, plain, under the lock)
 //   word 96 : reader entry counter        (line 3, atomic)
 constexpr int kLockWords = 128;
-constexpr int kLkHead = 0, kLkCount = 1, kLkWaiting = 2, kLkMax = 3;
+constexpr int kLkHead = 0, kLkMax = 3;
 constexpr int kLkTail = 32, kLkCs = 64, kLkRd = 96;
 
 // Barrier block: 64 words per level instance (count @0, generation @32).
@@ -44,50 +43,59 @@ struct Backoff {
 // Loads are batched 16 lines at a time; within one thread all addresses
 // are distinct and no other thread touches them, so the reordering keeps
 // the reference's semantics exactly.
+// SC1 = true: loads are agent-scope relaxed atomics (global_load ... sc1),
+// which bypass the CU's L1; used by the XCD-aware barrier, where other
+// workgroups' data is made visible per XCD rather than per workgroup.
+template <bool SC1>
+__device__ __forceinline__ int32_t ld_data(const int32_t *p) {
+  if (SC1) return __hip_atomic_load(p, __ATOMIC_RELAXED, HS_AGENT);
+  return *p;
+}
+
 // N source lines starting at `lo`: N independent loads, then N stores.
-template <int MODE, int N>
-__device__ __forceinline__ void shift_chunk(int32_t *__restrict__ col, int lo) {
+template <int MODE, int N, bool SC1>
+__device__ __forceinline__ void shift_chunk_x(int32_t *__restrict__ col, int lo) {
   int32_t v[N];
 #pragma unroll
   for (int k = 0; k < N; ++k)
-    v[k] = col[(lo + k + (MODE == 0 ? 0 : 1)) * kWordsPerLine];
+    v[k] = ld_data<SC1>(col + (lo + k + (MODE == 0 ? 0 : 1)) * kWordsPerLine);
 #pragma unroll
   for (int k = N - 1; k >= 0; --k)
     col[(lo + k + 1) * kWordsPerLine] = MODE == 0 ? v[k] : v[k] + 1;
 }
 
-template <int MODE>
+template <int MODE, bool SC1 = false>
 __device__ __forceinline__ void access_data(int32_t *__restrict__ s,
                                             long base_line, int off, int L) {
   int32_t *col = s + base_line * kWordsPerLine + off;
   int hi = L;  // source lines [0, hi) remain, processed from the top down
-  for (; hi >= 16; hi -= 16) shift_chunk<MODE, 16>(col, hi - 16);
+  for (; hi >= 16; hi -= 16) shift_chunk_x<MODE, 16, SC1>(col, hi - 16);
   switch (hi) {
-    case 15: shift_chunk<MODE, 15>(col, 0); break;
-    case 14: shift_chunk<MODE, 14>(col, 0); break;
-    case 13: shift_chunk<MODE, 13>(col, 0); break;
-    case 12: shift_chunk<MODE, 12>(col, 0); break;
-    case 11: shift_chunk<MODE, 11>(col, 0); break;
-    case 10: shift_chunk<MODE, 10>(col, 0); break;
-    case 9: shift_chunk<MODE, 9>(col, 0); break;
-    case 8: shift_chunk<MODE, 8>(col, 0); break;
-    case 7: shift_chunk<MODE, 7>(col, 0); break;
-    case 6: shift_chunk<MODE, 6>(col, 0); break;
-    case 5: shift_chunk<MODE, 5>(col, 0); break;
-    case 4: shift_chunk<MODE, 4>(col, 0); break;
-    case 3: shift_chunk<MODE, 3>(col, 0); break;
-    case 2: shift_chunk<MODE, 2>(col, 0); break;
-    case 1: shift_chunk<MODE, 1>(col, 0); break;
+    case 15: shift_chunk_x<MODE, 15, SC1>(col, 0); break;
+    case 14: shift_chunk_x<MODE, 14, SC1>(col, 0); break;
+    case 13: shift_chunk_x<MODE, 13, SC1>(col, 0); break;
+    case 12: shift_chunk_x<MODE, 12, SC1>(col, 0); break;
+    case 11: shift_chunk_x<MODE, 11, SC1>(col, 0); break;
+    case 10: shift_chunk_x<MODE, 10, SC1>(col, 0); break;
+    case 9: shift_chunk_x<MODE, 9, SC1>(col, 0); break;
+    case 8: shift_chunk_x<MODE, 8, SC1>(col, 0); break;
+    case 7: shift_chunk_x<MODE, 7, SC1>(col, 0); break;
+    case 6: shift_chunk_x<MODE, 6, SC1>(col, 0); break;
+    case 5: shift_chunk_x<MODE, 5, SC1>(col, 0); break;
+    case 4: shift_chunk_x<MODE, 4, SC1>(col, 0); break;
+    case 3: shift_chunk_x<MODE, 3, SC1>(col, 0); break;
+    case 2: shift_chunk_x<MODE, 2, SC1>(col, 0); break;
+    case 1: shift_chunk_x<MODE, 1, SC1>(col, 0); break;
     default: break;
   }
 }
 
 // Lane `t` of a WG whose data region starts at global thread id `tid0`.
-template <int MODE>
+template <int MODE, bool SC1 = false>
 __device__ __forceinline__ void wg_access(int32_t *s, long tid0, int L) {
   const long tid = tid0 + threadIdx.x;
-  access_data<MODE>(s, (tid / kWordsPerLine) * (L + 1),
-                    threadIdx.x % kWordsPerLine, L);
+  access_data<MODE, SC1>(s, (tid / kWordsPerLine) * (L + 1),
+                         threadIdx.x % kWordsPerLine, L);
 }
 
 // Semaphore reader body (accessData_semRd, HM:89-100): L loads, summed
@@ -188,35 +196,43 @@ __device__ __forceinline__ void mutex_unlock(uint32_t *lk, uint32_t *ring,
 }
 
 // ---------------------------------------------------------------------------
-// Reader/writer semaphore (HSS:32-232, HSE:32-249) over one record:
-// {lock, count, writerWaiting, max}.  Writers need count == max and take
-// the whole count; readers need !writerWaiting && count > 1 and take one.
+// Reader/writer semaphore (HSS:32-232, HSE:32-249).  The reference keeps
+// {count, writerWaiting} behind a CAS lock word and updates them with plain
+// accesses; here both live in ONE atomic word (count in bits 0-15,
+// writerWaiting in bit 16) and every transition is a single CAS, so a failed
+// attempt costs one RMW instead of a lock hand-off with a release/acquire
+// pair.  The state machine is the reference's: writers need count == max and
+// take the whole count, clearing writerWaiting; a failing writer sets
+// writerWaiting; readers need !writerWaiting && count > 1 and take one;
+// Post adds max (writer) or 1 (reader).
+constexpr uint32_t kSemWaitBit = 1u << 16;
+
+__device__ __forceinline__ uint32_t sem_init_word(uint32_t max) { return max | kSemWaitBit; }
+
 // Returns 1 = entered, 0 = not yet, -1 = aborted.
-__device__ __forceinline__ int sem_try_wait(uint32_t *lk, bool writer,
-                                            Guard &g) {
-  if (!(ld_rlx(lk + kLkHead) == 0 && cas_acq(lk + kLkHead, 0, 1) == 0))
-    return g.ok() ? 0 : -1;
-  const uint32_t cnt = ld_rlx(lk + kLkCount);
+__device__ __forceinline__ int sem_try_wait(uint32_t *lk, bool writer, Guard &g) {
   const uint32_t mx = ld_rlx(lk + kLkMax);
-  bool ok;
-  if (writer) {
-    ok = cnt == mx;
-    if (ok) {
-      st_rlx(lk + kLkCount, cnt - mx);
-      st_rlx(lk + kLkWaiting, 0);
+  uint32_t s = ld_rlx(lk + kLkHead);
+  for (;;) {
+    const uint32_t cnt = s & 0xffffu;
+    const bool waiting = (s & kSemWaitBit) != 0;
+    uint32_t want;
+    bool enter;
+    if (writer) {
+      enter = cnt == mx;
+      if (enter) want = 0;                       // count -= max, writerWaiting = 0
+      else if (!waiting) want = s | kSemWaitBit;  // announce the waiting writer
+      else return g.ok() ? 0 : -1;
     } else {
-      st_rlx(lk + kLkWaiting, 1);
+      enter = !waiting && cnt > 1;
+      if (!enter) return g.ok() ? 0 : -1;
+      want = s - 1;
     }
-  } else {
-    ok = ld_rlx(lk + kLkWaiting) == 0 && cnt > 1;
-    if (ok) st_rlx(lk + kLkCount, cnt - 1);
+    const uint32_t old = enter ? cas_acq(lk + kLkHead, s, want) : cas_rlx(lk + kLkHead, s, want);
+    if (old == s) return enter ? 1 : (g.ok() ? 0 : -1);
+    s = old;  // raced with another update: re-evaluate
+    if (!g.ok()) return -1;
   }
-  st_rel(lk + kLkHead, 0);
-  if (ok) {
-    fence_acq();  // the CS data written by the previous holder
-    return 1;
-  }
-  return g.ok() ? 0 : -1;
 }
 
 template <bool EBO>
@@ -240,18 +256,10 @@ __device__ __forceinline__ bool sem_wait(uint32_t *lk, bool writer,
   }
 }
 
+// Post: one release RMW (also publishes a writer's CS data).
 __device__ __forceinline__ bool sem_post(uint32_t *lk, bool writer,
-                                         const Backoff &bo, Guard &g) {
-  uint32_t lin = 0;
-  for (;;) {
-    if (ld_rlx(lk + kLkHead) == 0 && cas_acq(lk + kLkHead, 0, 1) == 0) break;
-    if (!g.ok()) return false;
-    lin = (lin + 1) % 31;  // linear backoff of HSS:202-203
-    sleep_quanta(lin > bo.poll_q ? bo.poll_q : lin);
-  }
-  const uint32_t cnt = ld_rlx(lk + kLkCount);
-  st_rlx(lk + kLkCount, writer ? cnt + ld_rlx(lk + kLkMax) : cnt + 1);
-  st_rel(lk + kLkHead, 0);  // also publishes the writer's CS data
+                                         const Backoff &, Guard &) {
+  add_rel(lk + kLkHead, writer ? ld_rlx(lk + kLkMax) : 1u);
   return true;
 }
 
@@ -363,6 +371,177 @@ __device__ __forceinline__ bool flag_tree_barrier(const FlagBar &f, int grp,
     uint32_t *lout = f.lout + (long)grp * f.per_cu;
     __syncthreads();
     for (int m = 1 + tid; m < f.per_cu; m += nthr) st_rel(lout + m, e_local);
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// XCD-aware tree barriers.
+//
+// MI355X has 8 XCDs, each with its own L2; agent-scope release/acquire
+// (buffer_wbl2 sc1 / buffer_inv sc1) costs ~56 ns and these operations are
+// serialised per XCD, so a barrier where each of 1024 WGs releases and
+// acquires spends ~20 us per episode in cache maintenance alone.  Workgroups
+// are dispatched round-robin over XCDs (WG b on XCD b % 8, verified at kernel
+// start through the XCC_ID hardware register), so every reference group
+// (blockIdx % NUM_CU) lies inside one XCD.  The barrier therefore has three
+// levels -- group, XCD, device -- and only the XCD leader touches the caches:
+//   * below the XCD leader, arrivals are relaxed atomics issued after the
+//     arriving WG's stores have completed to the shared L2 (s_waitcnt 0),
+//     and waiters read shared data with L1-bypassing agent-scope loads;
+//   * the XCD leader writes the XCD's L2 back (release) before arriving at
+//     the device level and invalidates it (acquire) before releasing its XCD.
+// Per episode this is 2 cache operations per XCD instead of 2-3 per WG.
+__device__ __forceinline__ void wait_mem() { __builtin_amdgcn_s_waitcnt(0); }
+
+__device__ __forceinline__ bool poll_ne(const uint32_t *p, uint32_t v,
+                                        uint32_t poll_q, Guard &g) {
+  while (ld_rlx(p) == v) {
+    if (!g.ok()) return false;
+    sleep_quanta(poll_q);
+  }
+  return true;
+}
+__device__ __forceinline__ bool poll_eq(const uint32_t *p, uint32_t v,
+                                        uint32_t poll_q, Guard &g) {
+  while (ld_rlx(p) != v) {
+    if (!g.ok()) return false;
+    sleep_quanta(poll_q);
+  }
+  return true;
+}
+
+__device__ __forceinline__ uint32_t xcc_id() {
+  return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 0xfu;  // HW_REG_XCC_ID
+}
+
+__device__ __forceinline__ int groups_on_xcd(int nab, int nxcd, int x) {
+  return nab / nxcd + (x < nab % nxcd ? 1 : 0);
+}
+
+struct XcdTree {
+  uint32_t *glob;      // kBarWords: count @0, gen @32 (one arrival per XCD)
+  uint32_t *xcd;       // nxcd * kBarWords
+  uint32_t *grp;       // nab * kBarWords
+  uint32_t *episodes;
+  int nab, per_cu, nxcd;
+};
+
+// XCD level and above; called by the last arriver of a group.
+__device__ __forceinline__ bool xcd_tree_upper(const XcdTree &t, int grp,
+                                               uint32_t poll_q, Guard &g) {
+  const int x = grp % t.nxcd;
+  uint32_t *cnt = t.xcd + (long)x * kBarWords, *gen = cnt + 32;
+  const uint32_t xg = ld_rlx(gen);
+  if (add_rlx(cnt, 1) != (uint32_t)groups_on_xcd(t.nab, t.nxcd, x) - 1)
+    return poll_ne(gen, xg, poll_q, g);
+  st_rlx(cnt, 0);
+  // XCD leader: publish the whole XCD, then the device level.
+  fence_rel();
+  const int nx = t.nab < t.nxcd ? t.nab : t.nxcd;
+  uint32_t *gc = t.glob, *ggen = t.glob + 32;
+  const uint32_t gg = ld_rlx(ggen);
+  bool ok = true;
+  if (add_acqrel(gc, 1) == (uint32_t)nx - 1) {
+    st_rlx(gc, 0);
+    st_rlx(t.episodes, ld_rlx(t.episodes) + 1);
+    st_rel(ggen, gg + 1);
+  } else {
+    ok = spin_until_ne(ggen, gg, poll_q, g);  // ends with the XCD's invalidate
+  }
+  wait_mem();
+  st_rlx(gen, xg + 1);
+  return ok;
+}
+
+// Master thread; every wave of the WG has run wait_mem() + __syncthreads().
+__device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp,
+                                                bool local_only,
+                                                uint32_t poll_q, Guard &g) {
+  if (t.per_cu > 1) {
+    uint32_t *cnt = t.grp + (long)grp * kBarWords, *gen = cnt + 32;
+    const uint32_t lg = ld_rlx(gen);
+    if (add_rlx(cnt, 1) != (uint32_t)t.per_cu - 1) return poll_ne(gen, lg, poll_q, g);
+    st_rlx(cnt, 0);
+    const bool ok = local_only ? true : xcd_tree_upper(t, grp, poll_q, g);
+    st_rlx(gen, lg + 1);
+    return ok;
+  }
+  return local_only ? true : xcd_tree_upper(t, grp, poll_q, g);
+}
+
+struct XcdFlags {
+  uint32_t *gin, *gout;  // nxcd: device level (XCD leaders)
+  uint32_t *xin, *xout;  // nab : XCD level (group masters), index = group
+  uint32_t *lin, *lout;  // nab * per_cu: group level, group-major
+  uint32_t *episodes;
+  int nab, per_cu, nxcd;
+};
+
+// Whole-WG call.  el = group-level epoch, eg = episode (XCD + device level).
+__device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int q,
+                                                 uint32_t el, uint32_t eg,
+                                                 bool local_only, uint32_t poll_q,
+                                                 Guard &g, int *s_flag) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  wait_mem();
+  __syncthreads();
+  if (f.per_cu > 1) {
+    uint32_t *lin = f.lin + (long)grp * f.per_cu, *lout = f.lout + (long)grp * f.per_cu;
+    if (q != 0) {
+      if (tid == 0) {
+        st_rlx(lin + q, el);
+        *s_flag = poll_eq(lout + q, el, poll_q, g);
+      }
+      __syncthreads();
+      return *s_flag != 0;
+    }
+    bool ok = true;
+    for (int m = 1 + tid; m < f.per_cu; m += nthr) ok = ok && poll_eq(lin + m, el, poll_q, g);
+    if (!__syncthreads_and(ok)) return false;
+  }
+  if (!local_only) {
+    const int x = grp % f.nxcd, k = grp / f.nxcd;
+    const int ngx = groups_on_xcd(f.nab, f.nxcd, x);
+    if (k != 0) {
+      if (tid == 0) {
+        st_rlx(f.xin + grp, eg);
+        *s_flag = poll_eq(f.xout + grp, eg, poll_q, g);
+      }
+      __syncthreads();
+      if (!*s_flag) return false;
+    } else {
+      bool ok = true;
+      for (int j = 1 + tid; j < ngx; j += nthr) ok = ok && poll_eq(f.xin + x + j * f.nxcd, eg, poll_q, g);
+      if (!__syncthreads_and(ok)) return false;
+      if (tid == 0) {  // XCD leader
+        const int nx = f.nab < f.nxcd ? f.nab : f.nxcd;
+        fence_rel();
+        bool lok = true;
+        if (x != 0) {
+          st_rlx(f.gin + x, eg);
+          lok = spin_until_eq(f.gout + x, eg, poll_q, g);
+        } else {
+          for (int y = 1; y < nx && lok; ++y) lok = spin_until_eq(f.gin + y, eg, poll_q, g);
+          if (lok) {
+            st_rlx(f.episodes, ld_rlx(f.episodes) + 1);
+            fence_rel();
+            for (int y = 1; y < nx; ++y) st_rlx(f.gout + y, eg);
+          }
+        }
+        if (nx == 1) fence_acq();
+        wait_mem();
+        *s_flag = lok;
+      }
+      __syncthreads();
+      if (!*s_flag) return false;
+      for (int j = 1 + tid; j < ngx; j += nthr) st_rlx(f.xout + x + j * f.nxcd, eg);
+    }
+  }
+  if (f.per_cu > 1) {
+    uint32_t *lout = f.lout + (long)grp * f.per_cu;
+    __syncthreads();
+    for (int m = 1 + tid; m < f.per_cu; m += nthr) st_rlx(lout + m, el);
   }
   return true;
 }

@@ -88,6 +88,10 @@ typedef struct {
   int backoff_max;   /* EBO cap mask (2^k-1); 0 = 1023 (MAX_BACKOFF-1) */
   int poll_sleep;    /* quanta between plain polls; <0 = 0, 0 = default */
   double timeout_s;  /* per-launch watchdog; 0 = 20 s */
+  int sync_scope;    /* 0 = XCD-aware barriers where the device has several
+                        XCDs (one cache writeback/invalidate per XCD and
+                        episode), 1 = agent-scope release/acquire by every
+                        WG (strict HIP memory model everywhere) */
 } hs_syncprim_cfg;
 
 typedef struct {
@@ -99,6 +103,7 @@ typedef struct {
   int state_ok;         /* 1 if every lock/barrier word is back in its
                            idle state (see hs_syncprim_run) */
   uint32_t lock_words[8]; /* first lock block / barrier words, for tests */
+  int sync_scope_used;  /* 0 = XCD-aware, 1 = strict */
 } hs_syncprim_result;
 
 /* Runs one microbenchmark exactly as the reference's ROI does: `reps`
@@ -212,6 +217,13 @@ typedef struct {
 int hs_batch_run(hs_ctx *ctx, const hs_batch_instance *inst, int n,
                  const hs_batch_cfg *cfg, int32_t *const *storages,
                  int64_t *entries, hs_batch_result *res);
+
+/* Autotuner for hs_batch_run: times a calibration copy of the batch (CS
+ * iterations capped at 20, one launch) over workers per CU {2,4,8,16} x
+ * poll interval {0,1,4} x EBO cap {63,255,1023} and writes the fastest
+ * persistent_wgs / poll_sleep / backoff_max into *cfg. */
+int hs_batch_autotune(hs_ctx *ctx, const hs_batch_instance *inst, int n,
+                      hs_batch_cfg *cfg, double *best_ms);
 
 typedef struct {
   int seed;

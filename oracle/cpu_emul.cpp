@@ -306,3 +306,41 @@ extern "C" double hso_cpu_run(int prim, int L, int num_wgs, int iters,
   if (cs_count) *cs_count = cs.load();
   return std::chrono::duration<double>(t1 - t0).count();
 }
+
+extern "C" double hso_cpu_barrier_run(int prim, int L, int num_wgs, int iters, int reps,
+                                      int wg, int num_cu, int mode, int threads,
+                                      int32_t *s, long n) {
+  const long N = hso_num_storage_locs(prim, num_wgs, num_cu, wg, L);
+  if (N < 0 || N != n || prim < 0 || prim > 3 || threads < 1) return -1.0;
+  if (num_wgs > num_cu && num_wgs % num_cu) return -1.0;
+  for (long i = 0; i < n; ++i) s[i] = (int32_t)i;
+  const int nab = num_wgs < num_cu ? num_wgs : num_cu;
+  const int perCU = ceil_div(num_wgs, nab);
+  const bool lx = (prim == 1 || prim == 3) && perCU > 1;
+  const int T = threads < num_wgs ? threads : num_wgs;
+  GenBarrier bar;
+  bar.n = (uint32_t)T;
+  auto worker = [&](int t) {
+    const int b0 = (int)((long)num_wgs * t / T), b1 = (int)((long)num_wgs * (t + 1) / T);
+    std::vector<int> cur(b1 - b0);
+    for (int b = b0; b < b1; ++b) cur[b - b0] = b;
+    for (int e = 0; e < iters * reps; ++e) {
+      for (int b = b0; b < b1; ++b) wg_access(s, (long)cur[b - b0] * wg, wg, L, mode);
+      if (lx) {  // local barrier (here: all threads), then the neighbour's region
+        bar.arrive();
+        for (int b = b0; b < b1; ++b) {
+          cur[b - b0] = (cur[b - b0] + nab) % num_wgs;
+          wg_access(s, (long)cur[b - b0] * wg, wg, L, mode);
+        }
+      }
+      bar.arrive();
+      for (int b = b0; b < b1; ++b) cur[b - b0] = (cur[b - b0] + 1) % num_wgs;
+    }
+  };
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t) th.emplace_back(worker, t);
+  for (auto &x : th) x.join();
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}

@@ -141,6 +141,15 @@ double hso_cpu_run(int prim, int num_ldst, int num_wgs, int iters, int reps,
                    int wg_size, int num_cu, int mode, int32_t *storage,
                    long n, int64_t *cs_count);
 
+/* Barrier workload (prims 0-3) on `threads` host threads, each owning a
+ * contiguous block of the num_wgs emulated WGs: per episode every thread
+ * runs its WGs' critical-section walks (same region rotation as the
+ * kernels), then all threads meet in one barrier.  Final storage equals
+ * the closed form.  Returns seconds, or < 0 for an unsupported shape. */
+double hso_cpu_barrier_run(int prim, int num_ldst, int num_wgs, int iters,
+                           int reps, int wg_size, int num_cu, int mode,
+                           int threads, int32_t *storage, long n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,8 @@ def _load_oracle():
     lib.hso_cpu_run.restype = ctypes.c_double
     lib.hso_cpu_run.argtypes = [ctypes.c_int] * 8 + [P(ctypes.c_int32), ctypes.c_long,
                                                      P(ctypes.c_int64)]
+    lib.hso_cpu_barrier_run.restype = ctypes.c_double
+    lib.hso_cpu_barrier_run.argtypes = [ctypes.c_int] * 9 + [P(ctypes.c_int32), ctypes.c_long]
     return lib
 
 

@@ -358,3 +358,14 @@ def test_uts_reference_tree_invariants(oracle):
                             ca.ctypes.data_as(P32), cb.ctypes.data_as(P32),
                             ctypes.byref(lg), ctypes.byref(ls))
     assert np.array_equal(val, val2)
+
+
+@pytest.mark.parametrize("prim", [0, 1, 2, 3])
+def test_cpu_barrier_pool_matches_closed_form(oracle, ohelp, prim):
+    """The threaded CPU baseline of the barrier workload lands on the closed form."""
+    for nw, ncu, threads, mode in [(16, 4, 3, 1), (8, 8, 4, 0), (32, 8, 5, 1)]:
+        want = ohelp.golden(prim, 3, nw, 4, 2, 64, ncu, mode)
+        got = np.zeros_like(want)
+        t = oracle.hso_cpu_barrier_run(prim, 3, nw, 4, 2, 64, ncu, mode, threads,
+                                       got.ctypes.data_as(P32), got.size)
+        assert t >= 0 and np.array_equal(got, want), (prim, nw, ncu, threads, mode)

@@ -1,0 +1,86 @@
+"""GPU parity of the relaxed-atomics kernels against the oracle's exact
+invariants: seqlock sequence numbers (closed form) and no torn snapshot;
+flags' dirty count and output rows; reference counters back to zero with a
+release recorded for every counter; split counters exact, reader totals in
+range."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def seq_expected(oracle, nw, gs, period, iters, n):
+    out = np.zeros(n, np.uint32)
+    oracle.hso_seqlock_expected(nw, gs, period, iters,
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n)
+    return out
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("nw,wg,gs,period,iters", [
+    (512, 128, 4, 16, 10),     # the reference's run file: ./seqlocks 512 128 4 {0,1}
+    (256, 64, 16, 65, 200),
+    (1024, 256, 16, 65, 50),   # full-occupancy shape, 1 writer WG per 64 readers
+])
+def test_seqlock(dev, oracle, variant, nw, wg, gs, period, iters):
+    seq, out, torn, r = dev.seqlock(nw, wg, gs, period, iters, variant)
+    want = seq_expected(oracle, nw, gs, period, iters, len(seq))
+    assert np.array_equal(seq[:gs], want[:gs])
+    assert all(int(x) % 2 == 0 for x in seq)
+    if variant == 0:
+        assert torn == 0
+        # every reader snapshot is r1 + r2 with r1 == r2 == a writer's even value
+        readers = np.array([b for b in range(nw) if b % period], np.int64)
+        o = out.reshape(nw, wg)[readers]
+        assert np.all(o % 4 == 0) and np.all(o >= 0)
+    assert r.aborted == 0
+
+
+@pytest.mark.parametrize("ntb,tb,accs,reps,cut,iters", [
+    (160, 64, 16, 4, 0.33, 1),      # the reference's run file (tbSize 32 -> wave64)
+    (256, 64, 16, 4, 0.2, 100),
+    (1024, 256, 16, 4, 0.33, 200),
+])
+def test_flags(dev, oracle, ntb, tb, accs, reps, cut, iters):
+    rnd = np.zeros(ntb * iters, np.float32)
+    oracle.hso_flags_rand(rnd.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), rnd.size, 2018)
+    want = oracle.hso_flags_dirty(rnd.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ntb, iters,
+                                  ctypes.c_float(cut))
+    out, dirty, r = dev.flags(ntb, tb, accs, reps, cut, iters, rnd)
+    assert dirty == want
+    o = out.reshape(ntb, tb)
+    assert np.all(o[:-1] == 0)
+    assert np.all(o[-1] == (reps * accs * accs if dirty else 0))
+
+
+def test_flags_never_dirty(dev):
+    rnd = np.ones(64 * 10, np.float32)  # all > cutoff
+    out, dirty, _ = dev.flags(64, 64, 8, 2, 0.5, 10, rnd)
+    assert dirty == 0 and not out.any()
+
+
+@pytest.mark.parametrize("sh,ntb,tb,reps,spacing", [
+    (32, 64, 32, 4, -1),            # the reference's run file
+    (32, 1024, 256, 100, 8),
+    (4, 512, 64, 50, 0),
+])
+def test_refcounter(dev, sh, ntb, tb, reps, spacing):
+    c0, c1, d0, d1, r = dev.refcounter(sh, ntb, tb, reps, spacing)
+    assert not c0.any() and not c1.any()
+    groups = ntb // sh
+    for d in (d0, d1):
+        per_counter = d.reshape(ntb, tb).reshape(sh, groups, tb).max(axis=0)
+        assert np.all(per_counter == 1)  # every counter saw its last release
+    assert r.atomic_ops == 4 * ntb * tb * reps
+
+
+@pytest.mark.parametrize("ntb,tb,iters", [(112, 128, 1), (256, 256, 1000), (1024, 256, 200)])
+def test_splitcounter(dev, ntb, tb, iters):
+    counters, out, r = dev.splitcounter(ntb, tb, iters)
+    assert np.all(counters == iters)
+    total = iters * counters.size
+    assert out.max() <= total
+    w = out.reshape(ntb, tb // 64, 64)
+    assert np.all(w[:, 0::2, :] == 0)  # writer waves write no output

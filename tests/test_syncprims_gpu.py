@@ -59,19 +59,22 @@ def test_uniq_semaphore(dev, ohelp, prim):
 
 @pytest.mark.parametrize("prim", BARRIER)
 @pytest.mark.parametrize("mode", [0, 1])
-def test_barrier_small(dev, ohelp, prim, mode):
+@pytest.mark.parametrize("scope", [0, 1])
+def test_barrier_small(dev, ohelp, prim, mode, scope):
     ncu = dev.info.num_cu
-    check(dev, ohelp, prim, 4, 64, 20, mode=mode)
-    check(dev, ohelp, prim, 4, 4 * ncu, 20, mode=mode)
-    check(dev, ohelp, prim, 2, 2 * ncu, 3, reps=2, mode=mode, wg=128)
+    for nw in (1, 5, 64, ncu, 4 * ncu):
+        r = check(dev, ohelp, prim, 4, nw, 20, mode=mode, sync_scope=scope)
+        assert r.sync_scope_used == scope
+    check(dev, ohelp, prim, 2, 2 * ncu, 3, reps=2, mode=mode, wg=128, sync_scope=scope)
 
 
 @pytest.mark.parametrize("prim", BARRIER)
-def test_barrier_full_config(dev, ohelp, prim):
+@pytest.mark.parametrize("scope", [0, 1])
+def test_barrier_full_config(dev, ohelp, prim, scope):
     """1024 WGs x 256 threads, L=10, 1000 iterations x 10 launches."""
     ncu = dev.info.num_cu
-    r = check(dev, ohelp, prim, 10, 4 * ncu, 1000, wg=256, mode=1)
-    assert r.episodes == 10000
+    r = check(dev, ohelp, prim, 10, 4 * ncu, 1000, wg=256, mode=1, sync_scope=scope)
+    assert r.episodes == 10000 and r.sync_scope_used == scope
 
 
 def test_fa_mutex_c1(dev, ohelp):

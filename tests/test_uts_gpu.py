@@ -1,0 +1,68 @@
+"""GPU parity of UTS work stealing against the oracle: the reference binary
+tree (uts.cu:290-339) gives exact leaves / leaf-sum / node counts, and
+geometric SHA-1 trees give the exact node, leaf and depth counts of the
+oracle's exhaustive search, including the published sample tree T1."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+P32 = ctypes.POINTER(ctypes.c_int32)
+
+
+def ref_tree(oracle, n, fert):
+    val = np.zeros(n, np.uint32)
+    ca = np.zeros(n, np.int32)
+    cb = np.zeros(n, np.int32)
+    lg, ls = ctypes.c_uint32(), ctypes.c_uint32()
+    oracle.hso_uts_ref_tree(n, fert, val.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                            ca.ctypes.data_as(P32), cb.ctypes.data_as(P32),
+                            ctypes.byref(lg), ctypes.byref(ls))
+    return val, ca, cb, lg.value, ls.value
+
+
+@pytest.mark.parametrize("n,fert,blocks", [(8193, 90, 0), (8193, 90, 128), (1, 50, 0),
+                                           (200001, 60, 0), (1000001, 95, 512)])
+def test_uts_reference_tree(dev, oracle, n, fert, blocks):
+    val, ca, cb, lg, ls = ref_tree(oracle, n, fert)
+    r = dev.uts_ref(val, ca, cb, blocks)
+    assert (r.nodes, r.leaves, r.leaf_sum) == (n, lg, ls)
+    assert r.aborted == 0
+
+
+def geo_count(oracle, seed, b0, d):
+    n, lv, md = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+    oracle.hso_uts_geo_count(seed, b0, d, ctypes.byref(n), ctypes.byref(lv), ctypes.byref(md))
+    return n.value, lv.value, md.value
+
+
+@pytest.mark.parametrize("seed,d,blocks", [(0, 1, 0), (1, 4, 0), (19, 6, 0), (7, 8, 64),
+                                           (123, 9, 0), (5, 0, 0)])
+def test_uts_geo_trees(dev, oracle, seed, d, blocks):
+    want = geo_count(oracle, seed, 4.0, d)
+    r = dev.uts_geo(seed, 4.0, d, blocks)
+    assert (r.nodes, r.leaves, r.max_depth) == want
+
+
+def test_uts_geo_T1(dev):
+    """UTS sample tree T1 (-t 1 -a 3 -d 10 -b 4 -r 19): 4130071 nodes, 3305118 leaves."""
+    r = dev.uts_geo(19, 4.0, 10)
+    assert (r.nodes, r.leaves, r.max_depth) == (4130071, 3305118, 10)
+
+
+def test_uts_serve_alone_equals_mixed(dev, oracle):
+    """Continuous batching: every tree's counts are the same alone and in the mix."""
+    reqs = []
+    rng = np.random.default_rng(5)
+    t = 0.0
+    for i in range(24):
+        t += rng.exponential(0.002)
+        reqs.append((int(rng.integers(0, 1 << 30)), int(rng.integers(3, 9)), 4.0, t))
+    resp, summ = dev.uts_serve(reqs, slots=8)
+    for (seed, d, b0, _), r in zip(reqs, resp):
+        assert (r.nodes, r.leaves, r.max_depth) == geo_count(oracle, seed, b0, d)
+        alone = dev.uts_geo(seed, b0, d)
+        assert (alone.nodes, alone.leaves, alone.max_depth) == (r.nodes, r.leaves, r.max_depth)
+    assert summ.total_nodes == sum(r.nodes for r in resp)
+    assert 0 < summ.p50_s <= summ.p99_s
