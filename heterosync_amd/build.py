@@ -46,6 +46,7 @@ CLIS = {
     "refCounter": "cli/ref_counter.cpp",
     "splitCounter": "cli/split_counter.cpp",
     "uts": "cli/uts.cpp",
+    "hs_probe": "cli/hs_probe.hip",
 }
 
 
@@ -78,8 +79,11 @@ def build(verbose: bool = False) -> str:
         if not os.path.exists(path):
             continue
         out = os.path.join(BIN, name)
-        _run([HIPCC, "-O2", "-std=c++17", path, "-o", out, f"-L{HERE}",
-              "-lheterosync_amd", "-Wl,-rpath,$ORIGIN/.."])
+        if src.endswith(".hip"):  # device code of its own
+            _run([HIPCC, *HIP_FLAGS, path, "-o", out])
+        else:
+            _run([HIPCC, "-O2", "-std=c++17", path, "-o", out, f"-L{HERE}",
+                  "-lheterosync_amd", "-Wl,-rpath,$ORIGIN/.."])
     if verbose:
         print("built", LIB)
     return LIB

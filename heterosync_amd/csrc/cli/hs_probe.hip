@@ -1,0 +1,238 @@
+// hs_probe.hip -- MI355X synchronization cost probe: the measured
+// denominators of the sync-path rooflines.
+//   1. workgroup -> XCD placement (XCC_ID hardware register)
+//   2. agent-scope release / acquire cost vs. number of concurrent WGs
+//   3. one-way flag hand-off latency (same XCD / across XCDs)
+//   4. device-scope atomic RMW throughput: one address, one per XCD, one per
+//      WG, one per lane (the uncontended peak)
+//   5. visibility: which store/load flavours observe each other across WGs
+//      on the same XCD and across XCDs
+// Every wait is bounded; prints one line per measurement.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../hs_syncprims_dev.h"
+
+using namespace hs;
+
+#define CHECK(x)                                                              \
+  do {                                                                        \
+    hipError_t e_ = (x);                                                      \
+    if (e_ != hipSuccess) {                                                   \
+      std::printf("HIP error %s at line %d\n", hipGetErrorString(e_), __LINE__); \
+      std::exit(1);                                                           \
+    }                                                                         \
+  } while (0)
+
+namespace {
+
+__global__ void k_xcc(uint32_t *out) {
+  if (threadIdx.x == 0) out[blockIdx.x] = xcc_id();
+}
+
+// mode 0: release fence; 1: acquire fence; 2: both; 3: both after dirtying 11 KB
+__global__ void k_fence(int mode, int iters, int32_t *data, uint32_t *sink) {
+  int32_t *mine = data + (long)blockIdx.x * 2816;
+  for (int i = 0; i < iters; ++i) {
+    if (mode == 3) {
+      for (int k = threadIdx.x; k < 2816; k += blockDim.x) mine[k] += 1;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      if (mode != 1) fence_rel();
+      if (mode != 0) fence_acq();
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sink[blockIdx.x] = mine[0];
+}
+
+__global__ void k_pingpong(int a, int b, int rounds, uint32_t *flag, uint32_t *res) {
+  if (threadIdx.x != 0) return;
+  const int me = blockIdx.x;
+  if (me != a && me != b) return;
+  const uint32_t parity = me == a ? 0 : 1;
+  uint32_t polls = 0;
+  for (int r = 0; r < rounds; ++r) {
+    const uint32_t want = 2 * r + parity;
+    while (ld_rlx(flag) != want)
+      if (++polls > (1u << 23)) {
+        res[2] = 1;
+        return;
+      }
+    st_rlx(flag, want + 1);
+  }
+}
+
+// 0: one address; 1: one per XCD; 2: one per WG; 3: one per lane
+__global__ void k_atomic(int mode, int iters, uint32_t *ctr) {
+  if (mode < 3 && threadIdx.x != 0) return;
+  uint32_t *p = ctr;
+  if (mode == 1) p = ctr + (blockIdx.x % 8) * 64;
+  if (mode == 2) p = ctr + blockIdx.x * 64;
+  if (mode == 3) p = ctr + blockIdx.x * blockDim.x + threadIdx.x;
+  for (int i = 0; i < iters; ++i) add_rlx(p, 1);
+}
+
+// writer WG a: stores (wmode 0 plain, 1 plain + release fence, 2 agent-scope
+// relaxed atomic stores), waits for completion, raises a flag; reader WG b:
+// loads (rmode 0 plain, 1 agent-scope relaxed atomic, 2 acquire fence +
+// plain) and counts values that are not the current round.
+__global__ void k_vis(int a, int b, int rounds, int wmode, int rmode, uint32_t *flag,
+                      int32_t *data, uint32_t *res) {
+  const int me = blockIdx.x;
+  if (me != a && me != b) return;
+  for (int r = 1; r <= rounds; ++r) {
+    if (me == a) {
+      if (threadIdx.x == 0) {
+        uint32_t n = 0;
+        while (ld_rlx(flag + 32) != (uint32_t)(r - 1))
+          if (++n > (1u << 22)) {
+            res[3] = 1;
+            break;
+          }
+      }
+      __syncthreads();
+      if (wmode == 2)
+        __hip_atomic_store(data + threadIdx.x * 16, r, __ATOMIC_RELAXED, HS_AGENT);
+      else
+        data[threadIdx.x * 16] = r;
+      wait_mem();
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        if (wmode == 1) fence_rel();
+        st_rlx(flag, r);
+      }
+    } else {
+      if (threadIdx.x == 0) {
+        uint32_t n = 0;
+        while (ld_rlx(flag) != (uint32_t)r)
+          if (++n > (1u << 22)) {
+            res[3] = 1;
+            break;
+          }
+      }
+      __syncthreads();
+      if (rmode == 2) fence_acq();
+      const int32_t v = rmode == 1 ? __hip_atomic_load(data + threadIdx.x * 16, __ATOMIC_RELAXED, HS_AGENT)
+                                   : data[threadIdx.x * 16];
+      if (v != r) atomicAdd(res + 1, 1u);
+      __syncthreads();
+      if (threadIdx.x == 0) st_rlx(flag + 32, r);
+    }
+  }
+}
+
+// Plain streaming copy of known size: the calibration reference for the
+// FETCH_SIZE / WRITE_SIZE hardware counters.
+__global__ void k_copy(const int4 *__restrict__ src, int4 *__restrict__ dst, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+float time_ms(const std::function<void()> &f) {
+  hipEvent_t e0, e1;
+  CHECK(hipEventCreate(&e0));
+  CHECK(hipEventCreate(&e1));
+  f();
+  CHECK(hipDeviceSynchronize());
+  CHECK(hipEventRecord(e0));
+  f();
+  CHECK(hipEventRecord(e1));
+  CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  CHECK(hipEventElapsedTime(&ms, e0, e1));
+  CHECK(hipEventDestroy(e0));
+  CHECK(hipEventDestroy(e1));
+  return ms;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc >= 2 && std::string(argv[1]) == "copy") {  // hs_probe copy <MiB>
+    const long mib = argc >= 3 ? std::atol(argv[2]) : 256;
+    const long n = mib * (1L << 20) / 16;
+    int4 *src, *dst;
+    CHECK(hipMalloc(&src, n * 16));
+    CHECK(hipMalloc(&dst, n * 16));
+    CHECK(hipMemset(src, 1, n * 16));
+    CHECK(hipDeviceSynchronize());
+    const float ms = time_ms([&] { k_copy<<<4096, 256>>>(src, dst, n); });
+    std::printf("copy %ld MiB: %.3f ms, %.1f GB/s (read+write)\n", mib, ms,
+                2.0 * n * 16 / (ms * 1e6));
+    return 0;
+  }
+  uint32_t *buf, *res;
+  int32_t *data;
+  CHECK(hipMalloc(&buf, 1 << 24));
+  CHECK(hipMalloc(&res, 4096));
+  CHECK(hipMalloc(&data, 4096L * 2816 * 4));
+  CHECK(hipMemset(data, 0, 4096L * 2816 * 4));
+  {
+    CHECK(hipMemset(buf, 0xff, 4096 * 4));
+    k_xcc<<<2048, 64>>>(buf);
+    CHECK(hipDeviceSynchronize());
+    std::vector<uint32_t> h(2048);
+    CHECK(hipMemcpy(h.data(), buf, 2048 * 4, hipMemcpyDeviceToHost));
+    int rr = 0;
+    for (int i = 0; i < 2048; ++i) rr += h[i] == (h[0] + i) % 8;
+    std::printf("placement: WG b on XCD (b + %u) mod 8 for %d/2048 WGs\n", h[0], rr);
+  }
+  for (int mode = 0; mode <= 3; ++mode)
+    for (int nwg : {1, 8, 256, 1024}) {
+      const int it = 200;
+      const float ms = time_ms([&] { k_fence<<<nwg, 256>>>(mode, it, data, buf); });
+      static const char *kName[] = {"release", "acquire", "release+acquire",
+                                    "release+acquire, 11 KB dirty"};
+      std::printf("fence %-30s %4d WGs: %8.3f us per WG-iteration\n", kName[mode], nwg,
+                  ms * 1000 / it);
+    }
+  for (auto pr : std::vector<std::pair<int, int>>{{0, 8}, {0, 1}, {0, 4}}) {
+    const int rounds = 2000;
+    CHECK(hipMemset(res, 0, 64));
+    const float ms = time_ms([&] {
+      CHECK(hipMemset(buf, 0, 256));
+      k_pingpong<<<16, 64>>>(pr.first, pr.second, rounds, buf, res);
+    });
+    uint32_t hr[4];
+    CHECK(hipMemcpy(hr, res, 16, hipMemcpyDeviceToHost));
+    std::printf("hand-off WG%d<->WG%d (%s): %.3f us one way%s\n", pr.first, pr.second,
+                pr.second == 8 ? "same XCD" : "other XCD", ms * 1000 / (2 * rounds),
+                hr[2] ? " (TIMEOUT)" : "");
+  }
+  for (int mode = 0; mode < 4; ++mode)
+    for (int nwg : {256, 1024}) {
+      const int it = mode == 3 ? 100 : 1000;
+      CHECK(hipMemset(buf, 0, 1 << 22));
+      const float ms = time_ms([&] { k_atomic<<<nwg, 256>>>(mode, it, buf); });
+      const double ops = (double)it * nwg * (mode == 3 ? 256 : 1);
+      static const char *kName[] = {"one address", "one per XCD", "one per WG", "one per lane"};
+      std::printf("atomic add %-12s %4d WGs: %9.3f G ops/s\n", kName[mode], nwg, ops / ms / 1e6);
+    }
+  for (int wmode = 0; wmode < 3; ++wmode)
+    for (int rmode = 0; rmode < 3; ++rmode)
+      for (int pr = 0; pr < 2; ++pr) {
+        const int a = 0, b = pr ? 1 : 8, rounds = 2000;
+        CHECK(hipMemset(buf, 0, 4096));
+        CHECK(hipMemset(res, 0, 64));
+        CHECK(hipMemset(data, 0, 1 << 20));
+        k_vis<<<16, 256>>>(a, b, rounds, wmode, rmode, buf, data, res);
+        CHECK(hipDeviceSynchronize());
+        uint32_t hr[4];
+        CHECK(hipMemcpy(hr, res, 16, hipMemcpyDeviceToHost));
+        static const char *kW[] = {"plain st", "plain st + release", "agent st"};
+        static const char *kR[] = {"plain ld", "agent ld", "acquire + plain ld"};
+        std::printf("visibility %-18s -> %-18s %s: stale %u / %d%s\n", kW[wmode], kR[rmode],
+                    pr ? "other XCD" : "same XCD ", hr[1], rounds * 256,
+                    hr[3] ? " (TIMEOUT)" : "");
+      }
+  std::printf("done\n");
+  return 0;
+}

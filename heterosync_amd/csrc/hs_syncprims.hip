@@ -39,6 +39,7 @@ struct SPArgs {
   Backoff bo;
   uint64_t ticks;
   uint32_t *xcd_map;  // per launch: physical XCD of each logical XCD (b % nxcd)
+  uint32_t *flat;     // sense-reversing family: main block, local block
 };
 
 template <int KIND, bool UNIQ, int MODE>
@@ -178,6 +179,74 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
   }
 }
 
+// Sense-reversing barrier family (main.hip.cpp:195-539, ids 37-42).
+//   VAR 0 SRB / 1 SRB + local exchange: tree (group -> device) when more
+//     than 4 WGs share a group, else flat over the grid
+//     (joinBarrier_helperSRB, hipLocksBarrierAtomicSRB.h:186-225);
+//   VAR 2 NaiveSRB / 3 + local exchange: flat over the grid's WGs;
+//   VAR 4 NaiveAllSRB / 5 + local exchange: flat over every thread
+//     (one arrival of weight 64 per wave).
+// The local-exchange barrier is the per-group barrier (VAR 1) or a second
+// flat barrier (VAR 3, 5).  VAR 3 re-visits its own region, because its
+// exchange stride is the grid (main.hip.cpp:472-474, 1568); VAR 1 and 5
+// step by the group count (main.hip.cpp:411-413, 513-515).  XCD selects
+// the XCD-aware tree and write-through data accesses; the host uses it
+// only for the tree form, flat barriers always release/acquire per arrival.
+template <int VAR, int MODE, bool XCD>
+__global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
+  __shared__ int s_ok;
+  Guard g;
+  g.init(a.ctl, a.ticks);
+  const int b = blockIdx.x, grid = gridDim.x, nab = a.nab;
+  const int grp = b % nab;
+  const bool lx = (VAR & 1) && a.per_cu > 1;
+  const bool all = VAR >= 4;
+  const bool tree = VAR <= 1 && a.per_cu > 4;
+  const uint64_t parts = all ? (uint64_t)grid * blockDim.x : (uint64_t)grid;
+  const bool wave_lead = (threadIdx.x & 63) == 0;
+  int cur = b;
+  if (is_master()) {
+    s_ok = !g.aborted();
+    if (XCD) {
+      const uint32_t me = xcc_id();
+      const uint32_t old = cas_rlx(a.xcd_map + b % a.nxcd, 0xffffffffu, me);
+      if (old != 0xffffffffu && old != me) {
+        st_rlx(a.ctl + 32, 1);
+        st_rlx(a.ctl, 1);
+        s_ok = 0;
+      }
+    }
+  }
+  __syncthreads();
+  if (!s_ok) return;
+  auto barrier = [&](bool local) -> bool {
+    wait_mem();
+    __syncthreads();
+    bool ok = true;
+    if ((local && VAR == 1) || (!local && tree)) {
+      if (is_master())
+        ok = XCD ? xcd_tree_arrive(a.xt, grp, local, a.bo.poll_q, g)
+                 : atomic_tree_arrive(a.tb, grp, local, a.bo.poll_q, g);
+    } else {
+      uint32_t *blk = a.flat + (local ? kBarWords : 0);
+      uint32_t *ep = local ? nullptr : a.tb.episodes;
+      if (all ? wave_lead : is_master())
+        ok = flat_arrive(blk, all ? 64u : 1u, parts, ep, a.bo.poll_q, g);
+    }
+    return __syncthreads_and(ok) != 0;
+  };
+  for (int i = 0; i < a.iters; ++i) {
+    wg_access<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
+    if (lx) {
+      if (!barrier(true)) return;
+      if (VAR != 3) cur = (cur + nab) % grid;
+      wg_access<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
+    }
+    if (!barrier(false)) return;
+    cur = (cur + 1) % grid;
+  }
+}
+
 // Initialises lock blocks and (sleep mutex) ring slots.
 __global__ void k_init_locks(uint32_t *locks, int nlocks, uint32_t sem_max,
                              uint32_t *ring, long ring_len, int stride) {
@@ -194,7 +263,7 @@ __global__ void k_init_locks(uint32_t *locks, int nlocks, uint32_t sem_max,
   }
 }
 
-bool is_barrier(int p) { return p < 4; }
+bool is_barrier(int p) { return p < 4 || (p >= 37 && p <= 42); }
 bool is_mutex(int p) { return (p >= 4 && p <= 7) || (p >= 20 && p <= 23); }
 bool is_sem(int p) { return (p >= 8 && p <= 15) || (p >= 24 && p <= 31); }
 
@@ -215,6 +284,12 @@ KernelFn pick_kernel(int p) {
     case 21: return k_mutex<kEBO, true, MODE>;
     case 22: return k_mutex<kSleep, true, MODE>;
     case 23: return k_mutex<kFA, true, MODE>;
+    case 37: return k_srb<0, MODE, XCD>;
+    case 38: return k_srb<1, MODE, XCD>;
+    case 39: return k_srb<2, MODE, XCD>;
+    case 40: return k_srb<3, MODE, XCD>;
+    case 41: return k_srb<4, MODE, XCD>;
+    case 42: return k_srb<5, MODE, XCD>;
     default: break;
   }
   if (p >= 8 && p <= 11) return k_sem<false, false, MODE>;
@@ -246,7 +321,14 @@ extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
                                int32_t *storage, long n, hs_syncprim_result *res) {
   if (!c || !cfg || !storage || !res) return HS_EINVAL;
   const int p = cfg->prim;
-  const bool xcd = cfg->sync_scope == 0 && p >= 0 && p < 4 && num_xcds(c) > 1;
+  // XCD-aware form: the tree barriers, and the SRB family where it is a tree
+  // (more than 4 WGs per group); flat barriers stay release/acquire.
+  const int ncu = c->num_cu;
+  const int per = cfg->num_wgs > 0 ? (cfg->num_wgs + std::min(cfg->num_wgs, ncu) - 1) /
+                                         std::min(cfg->num_wgs, ncu)
+                                   : 1;
+  const bool xcd_form = (p >= 0 && p < 4) || ((p == 37 || p == 38) && per > 4);
+  const bool xcd = cfg->sync_scope == 0 && xcd_form && num_xcds(c) > 1;
   if (!xcd) return run_once(c, cfg, storage, n, res, false, nullptr);
   std::vector<int32_t> init(storage, storage + (n > 0 ? n : 0));
   bool topo = false;
@@ -314,13 +396,15 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   if ((rc = arena.alloc(&d_locks, (size_t)nlocks * kLockWords))) return rc;
   if (sleep_mx && (rc = arena.alloc(&d_ring, (size_t)nlocks * ring_len * ring_stride)))
     return rc;
-  const size_t bar_words = (size_t)kBarWords * (1 + nxcd + nab) + HS_LINE_WORDS;
+  const size_t bar_words = (size_t)kBarWords * (1 + 2 * nxcd + nab) + HS_LINE_WORDS;
   if ((rc = arena.alloc(&d_bar, bar_words))) return rc;
   const size_t flag_words =
       2 * (size_t)nxcd + 2 * (size_t)nab + 2 * (size_t)nab * per + 4 * HS_LINE_WORDS;
   if ((rc = arena.alloc(&d_flags, flag_words))) return rc;
-  uint32_t *d_xmap;
+  uint32_t *d_xmap, *d_flat;
   if ((rc = arena.alloc(&d_xmap, (size_t)reps * nxcd))) return rc;
+  if ((rc = arena.alloc(&d_flat, 2 * (size_t)kBarWords))) return rc;
+  HS_HIP(hipMemsetAsync(d_flat, 0, 2 * kBarWords * 4, c->stream));
   HS_HIP(hipMemsetAsync(d_xmap, 0xff, (size_t)reps * nxcd * 4, c->stream));
   HS_HIP(hipMemsetAsync(d_ctl, 0, 64 * sizeof(uint32_t), c->stream));
   HS_HIP(hipMemsetAsync(d_locks, 0, (size_t)nlocks * kLockWords * 4, c->stream));
@@ -346,16 +430,18 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   a.ring = d_ring;
   a.ring_len = ring_len;
   a.ring_stride = ring_stride;
-  uint32_t *episodes = d_bar + kBarWords * (1 + nxcd + nab);
-  // strict: global block, nab group blocks; XCD: global, nxcd XCD blocks, nab groups
+  uint32_t *episodes = d_bar + kBarWords * (1 + 2 * nxcd + nab);
+  // strict: global block, nab group blocks;
+  // XCD: global, nxcd XCD blocks, nab group blocks, nxcd release words
   a.tb = TreeBar{d_bar, d_bar + kBarWords, episodes, nab, per};
-  a.xt = XcdTree{d_bar, d_bar + kBarWords, d_bar + kBarWords * (1 + nxcd), episodes, nab,
-                 per, nxcd};
+  a.xt = XcdTree{d_bar, d_bar + kBarWords, d_bar + kBarWords * (1 + nxcd),
+                 d_bar + kBarWords * (1 + nxcd + nab), episodes, nab, per, nxcd};
   uint32_t *fl = d_flags + 4 * HS_LINE_WORDS;
   a.fb = FlagBar{fl, fl + nab, fl + 2 * nab, fl + 2 * nab + (size_t)nab * per, episodes,
                  nab, per};
   a.xf = XcdFlags{fl, fl + nxcd, fl + 2 * nxcd, fl + 2 * nxcd + nab, fl + 2 * nxcd + 2 * nab,
                   fl + 2 * nxcd + 2 * nab + (size_t)nab * per, episodes, nab, per, nxcd};
+  a.flat = d_flat;
   a.nxcd = nxcd;
   a.L = L;
   a.iters = iters;
@@ -366,7 +452,9 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   a.bo.max_q = cfg->backoff_max > 0 ? cfg->backoff_max : 1023;
   a.bo.poll_q = cfg->poll_sleep < 0 ? 0 : (cfg->poll_sleep > 0 ? cfg->poll_sleep : 1);
   a.ticks = timeout_ticks(c, cfg->timeout_s);
-  const bool lx = p == 1 || p == 3;
+  const bool lx = p == 1 || p == 3 || p == 38 || p == 40 || p == 42;
+  const bool srb = p >= 37 && p <= 42;
+  const bool srb_tree = (p == 37 || p == 38) && per > 4;
   const uint32_t local_per_iter = (lx && per > 1) ? 2 : 1;
 
   // ---- region of interest (main.hip.cpp:2354-2511) -----------------------
@@ -411,8 +499,10 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   HS_HIP(hipMemcpy(ctlw, d_ctl, sizeof ctlw, hipMemcpyDeviceToHost));
   const uint32_t abort_w = ctlw[0];
   if (topology_fail) *topology_fail = ctlw[32] != 0;
-  std::vector<uint32_t> locks((size_t)nlocks * kLockWords), bar(bar_words), flags(flag_words);
+  std::vector<uint32_t> locks((size_t)nlocks * kLockWords), bar(bar_words), flags(flag_words),
+      flat(2 * kBarWords);
   HS_HIP(hipMemcpy(locks.data(), d_locks, locks.size() * 4, hipMemcpyDeviceToHost));
+  HS_HIP(hipMemcpy(flat.data(), d_flat, flat.size() * 4, hipMemcpyDeviceToHost));
   HS_HIP(hipMemcpy(bar.data(), d_bar, bar.size() * 4, hipMemcpyDeviceToHost));
   HS_HIP(hipMemcpy(flags.data(), d_flags, flags.size() * 4, hipMemcpyDeviceToHost));
   const uint64_t total_ep = (uint64_t)iters * reps;
@@ -450,19 +540,36 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
                            lk0[kLkTail], lk0[kLkCs], lk0[kLkRd], (uint32_t)nlocks};
     std::memcpy(res->lock_words, w, sizeof(w));
   } else {
-    const uint32_t ep = bar[kBarWords * (1 + nxcd + nab)];
+    const uint32_t ep = bar[kBarWords * (1 + 2 * nxcd + nab)];
     res->episodes = ep;
     ok = ok && ep == (uint32_t)total_ep;
     const uint32_t te = (uint32_t)total_ep, tl = (uint32_t)(total_ep * local_per_iter);
-    if (p <= 1) {
+    if (srb && !srb_tree) {
+      // flat barriers: monotonic 64-bit arrival count, generation = episodes
+      const uint64_t parts = (p >= 41) ? (uint64_t)nw * wg : (uint64_t)nw;
+      auto flat_ok = [&](int k) {
+        uint64_t cnt;
+        std::memcpy(&cnt, &flat[(size_t)k * kBarWords], 8);
+        return cnt == parts * total_ep && flat[(size_t)k * kBarWords + 32] == te;
+      };
+      ok = ok && flat_ok(0);
+      if ((p == 40 || p == 42) && per > 1) ok = ok && flat_ok(1);
+      if (p == 38 && per > 1)  // per-group local barrier (release/acquire form)
+        for (int gi = 0; gi < nab; ++gi)
+          ok = ok && bar[kBarWords * (1 + gi)] == 0 && bar[kBarWords * (1 + gi) + 32] == te;
+      const uint32_t w[8] = {flat[0], flat[1], flat[32], flat[kBarWords + 32], ep, 0, 0, 0};
+      std::memcpy(res->lock_words, w, sizeof(w));
+    } else if (p <= 1 || srb) {
       const size_t g0 = xcd_mode ? kBarWords * (1 + nxcd) : kBarWords;
       ok = ok && bar[0] == 0 && bar[32] == te;
       if (xcd_mode)
         for (int x = 0; x < std::min(nab, nxcd); ++x)
-          ok = ok && bar[kBarWords * (1 + x)] == 0 && bar[kBarWords * (1 + x) + 32] == te;
+          ok = ok && bar[kBarWords * (1 + x)] == 0 &&
+               bar[kBarWords * (1 + nxcd + nab + x)] == te;  // release copy
       if (per > 1)
         for (int gi = 0; gi < nab; ++gi)
-          ok = ok && bar[g0 + kBarWords * gi] == 0 && bar[g0 + kBarWords * gi + 32] == tl;
+          ok = ok && bar[g0 + kBarWords * gi] == 0 &&
+               bar[g0 + kBarWords * gi + 32] == (xcd_mode ? (lx ? te : 0u) : tl);
       const uint32_t w[8] = {bar[0], bar[32], bar[g0], bar[g0 + 32], ep, 0, 0, 0};
       std::memcpy(res->lock_words, w, sizeof(w));
     } else {

@@ -43,9 +43,10 @@ struct Backoff {
 // Loads are batched 16 lines at a time; within one thread all addresses
 // are distinct and no other thread touches them, so the reordering keeps
 // the reference's semantics exactly.
-// SC1 = true: loads are agent-scope relaxed atomics (global_load ... sc1),
-// which bypass the CU's L1; used by the XCD-aware barrier, where other
-// workgroups' data is made visible per XCD rather than per workgroup.
+// SC1 = true: loads and stores are agent-scope relaxed atomics
+// (global_load/store ... sc1): loads bypass the CU's L1 and stores are
+// written through, so the data is at the device coherence point once the
+// wave's memory counter drains.  Used by the XCD-aware barriers.
 template <bool SC1>
 __device__ __forceinline__ int32_t ld_data(const int32_t *p) {
   if (SC1) return __hip_atomic_load(p, __ATOMIC_RELAXED, HS_AGENT);
@@ -60,8 +61,13 @@ __device__ __forceinline__ void shift_chunk_x(int32_t *__restrict__ col, int lo)
   for (int k = 0; k < N; ++k)
     v[k] = ld_data<SC1>(col + (lo + k + (MODE == 0 ? 0 : 1)) * kWordsPerLine);
 #pragma unroll
-  for (int k = N - 1; k >= 0; --k)
-    col[(lo + k + 1) * kWordsPerLine] = MODE == 0 ? v[k] : v[k] + 1;
+  for (int k = N - 1; k >= 0; --k) {
+    const int32_t val = MODE == 0 ? v[k] : v[k] + 1;
+    if (SC1)  // write-through to the coherence point: nothing left dirty in L2
+      __hip_atomic_store(col + (lo + k + 1) * kWordsPerLine, val, __ATOMIC_RELAXED, HS_AGENT);
+    else
+      col[(lo + k + 1) * kWordsPerLine] = val;
+  }
 }
 
 template <int MODE, bool SC1 = false>
@@ -421,53 +427,54 @@ __device__ __forceinline__ int groups_on_xcd(int nab, int nxcd, int x) {
 
 struct XcdTree {
   uint32_t *glob;      // kBarWords: count @0, gen @32 (one arrival per XCD)
-  uint32_t *xcd;       // nxcd * kBarWords
-  uint32_t *grp;       // nab * kBarWords
+  uint32_t *xcd;       // nxcd * kBarWords: count @0, gen @32
+  uint32_t *grp;       // nab * kBarWords: count @0, gen @32 (local exchange)
+  uint32_t *rel;       // nxcd * kBarWords: per-XCD copy of the device generation
   uint32_t *episodes;
   int nab, per_cu, nxcd;
 };
 
-// XCD level and above; called by the last arriver of a group.
-__device__ __forceinline__ bool xcd_tree_upper(const XcdTree &t, int grp,
-                                               uint32_t poll_q, Guard &g) {
-  const int x = grp % t.nxcd;
-  uint32_t *cnt = t.xcd + (long)x * kBarWords, *gen = cnt + 32;
-  const uint32_t xg = ld_rlx(gen);
-  if (add_rlx(cnt, 1) != (uint32_t)groups_on_xcd(t.nab, t.nxcd, x) - 1)
-    return poll_ne(gen, xg, poll_q, g);
-  st_rlx(cnt, 0);
-  // XCD leader: publish the whole XCD, then the device level.
-  fence_rel();
-  const int nx = t.nab < t.nxcd ? t.nab : t.nxcd;
-  uint32_t *gc = t.glob, *ggen = t.glob + 32;
-  const uint32_t gg = ld_rlx(ggen);
-  bool ok = true;
-  if (add_acqrel(gc, 1) == (uint32_t)nx - 1) {
-    st_rlx(gc, 0);
-    st_rlx(t.episodes, ld_rlx(t.episodes) + 1);
-    st_rel(ggen, gg + 1);
-  } else {
-    ok = spin_until_ne(ggen, gg, poll_q, g);  // ends with the XCD's invalidate
-  }
-  wait_mem();
-  st_rlx(gen, xg + 1);
-  return ok;
-}
-
 // Master thread; every wave of the WG has run wait_mem() + __syncthreads().
+// Arrival climbs group -> XCD -> device; the device releaser writes one
+// generation word per XCD and every WG of that XCD polls its word, so the
+// release is a single hop.
 __device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp,
                                                 bool local_only,
                                                 uint32_t poll_q, Guard &g) {
-  if (t.per_cu > 1) {
+  if (local_only) {  // group-level barrier (local exchange, HM:352-353)
+    if (t.per_cu <= 1) return true;
     uint32_t *cnt = t.grp + (long)grp * kBarWords, *gen = cnt + 32;
     const uint32_t lg = ld_rlx(gen);
     if (add_rlx(cnt, 1) != (uint32_t)t.per_cu - 1) return poll_ne(gen, lg, poll_q, g);
     st_rlx(cnt, 0);
-    const bool ok = local_only ? true : xcd_tree_upper(t, grp, poll_q, g);
     st_rlx(gen, lg + 1);
-    return ok;
+    return true;
   }
-  return local_only ? true : xcd_tree_upper(t, grp, poll_q, g);
+  const int x = grp % t.nxcd;
+  uint32_t *rel = t.rel + (long)x * kBarWords;
+  const uint32_t rv = ld_rlx(rel);  // cannot advance before this WG arrives
+  if (t.per_cu > 1) {
+    uint32_t *cnt = t.grp + (long)grp * kBarWords;
+    if (add_rlx(cnt, 1) != (uint32_t)t.per_cu - 1) return poll_ne(rel, rv, poll_q, g);
+    st_rlx(cnt, 0);
+  }
+  uint32_t *xc = t.xcd + (long)x * kBarWords;
+  if (add_rlx(xc, 1) != (uint32_t)groups_on_xcd(t.nab, t.nxcd, x) - 1)
+    return poll_ne(rel, rv, poll_q, g);
+  st_rlx(xc, 0);
+  // XCD leader: publish whatever the XCD left dirty, then the device level.
+  fence_rel();
+  const int nx = t.nab < t.nxcd ? t.nab : t.nxcd;
+  uint32_t *gc = t.glob, *ggen = t.glob + 32;
+  const uint32_t gg = ld_rlx(ggen);
+  if (add_acqrel(gc, 1) == (uint32_t)nx - 1) {
+    st_rlx(gc, 0);
+    st_rlx(t.episodes, ld_rlx(t.episodes) + 1);
+    st_rel(ggen, gg + 1);
+    for (int y = 0; y < nx; ++y) st_rlx(t.rel + (long)y * kBarWords, rv + 1);
+    return true;
+  }
+  return poll_ne(rel, rv, poll_q, g);
 }
 
 struct XcdFlags {
@@ -522,8 +529,9 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
           st_rlx(f.gin + x, eg);
           lok = spin_until_eq(f.gout + x, eg, poll_q, g);
         } else {
-          for (int y = 1; y < nx && lok; ++y) lok = spin_until_eq(f.gin + y, eg, poll_q, g);
+          for (int y = 1; y < nx && lok; ++y) lok = poll_eq(f.gin + y, eg, poll_q, g);
           if (lok) {
+            fence_acq();
             st_rlx(f.episodes, ld_rlx(f.episodes) + 1);
             fence_rel();
             for (int y = 1; y < nx; ++y) st_rlx(f.gout + y, eg);
@@ -544,6 +552,32 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
     for (int m = 1 + tid; m < f.per_cu; m += nthr) st_rlx(lout + m, el);
   }
   return true;
+}
+
+// ---------------------------------------------------------------------------
+// Flat (single-level) barrier: the "naive" sense-reversing barriers
+// (hipLocksBarrierAtomicSRB.h:19-64, 227-271).  `weight` is the number of
+// participants one arrival stands for: 1 per WG, or a wave's 64 lanes when
+// every thread joins (NaiveAll: one RMW per wave instead of one per lane).
+// The arrival counter is 64-bit and never reset, so the episode number is
+// the arrival's own return value divided by the participant count -- no
+// separate read of the generation word before arriving and no reset store
+// that a fast WG of the next episode could overtake.  The last arriver
+// publishes generation e+1 with a release store; waiters acquire it.
+//   blk[0..1] arrival count (64-bit), blk[32] generation.
+__device__ __forceinline__ bool flat_arrive(uint32_t *blk, uint32_t weight, uint64_t parts,
+                                            uint32_t *episodes, uint32_t poll_q, Guard &g) {
+  uint64_t *cnt = reinterpret_cast<uint64_t *>(blk);
+  uint32_t *gen = blk + 32;
+  const uint64_t old =
+      __hip_atomic_fetch_add(cnt, (uint64_t)weight, __ATOMIC_ACQ_REL, HS_AGENT);
+  const uint32_t e = (uint32_t)(old / parts);
+  if ((old + weight) % parts == 0) {
+    if (episodes) st_rlx(episodes, ld_rlx(episodes) + 1);
+    st_rel(gen, e + 1);
+    return true;
+  }
+  return spin_until_eq(gen, e + 1, poll_q, g);
 }
 
 }  // namespace hs

@@ -32,6 +32,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -150,7 +152,12 @@ struct UtsArgs {
   const uint32_t *submitted;
   uint32_t *resp;          // per slot: 8 words (nodes lo/hi, leaves lo/hi, maxd, done seq)
   const uint32_t *stop;
+  // optional per-WG statistics (null: off), kStatWords each:
+  // busy ticks, idle ticks, batches, nodes, donations, donated, steals, stolen
+  unsigned long long *stats;
 };
+
+constexpr int kStatWords = 8;
 
 __device__ __forceinline__ uint32_t meta_slot(uint32_t m) { return m >> 16; }
 __device__ __forceinline__ uint32_t meta_depth(uint32_t m) { return m & 0xffffu; }
@@ -177,10 +184,9 @@ __device__ __forceinline__ bool pool_lock(uint32_t *lk, Guard &g) {
   }
 }
 
-template <int TREE>
-__device__ __forceinline__ uint32_t *rec_at(uint32_t *base, long i) {
-  return base + i * Rec<TREE>::W;
-}
+// Records are stored structure-of-arrays: word w of entry i of a deque or
+// pool with capacity `cap` is base[w * cap + i], so lane-parallel record
+// accesses are contiguous.
 
 template <int TREE, bool SERVE>
 __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
@@ -207,6 +213,28 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
   long top = 0, bot = 0;  // deque holds [bot, top)
   bool hungry = false;
   uint32_t batches = 0;
+  unsigned long long st_busy = 0, st_idle = 0, st_nodes = 0, st_don = 0, st_donr = 0,
+                     st_steal = 0, st_stolen = 0;
+  unsigned long long t_mark = a.stats ? wall_clock64() : 0;
+  // folds the time since the last mark into busy or idle
+  auto tick = [&](bool busy) {
+    if (!a.stats) return;
+    const unsigned long long now = wall_clock64();
+    (busy ? st_busy : st_idle) += now - t_mark;
+    t_mark = now;
+  };
+  auto flush_stats = [&]() {
+    if (!a.stats || lane != 0) return;
+    unsigned long long *o = a.stats + (long)blockIdx.x * kStatWords;
+    o[0] = st_busy;
+    o[1] = st_idle;
+    o[2] = batches;
+    o[3] = st_nodes;
+    o[4] = st_don;
+    o[5] = st_donr;
+    o[6] = st_steal;
+    o[7] = st_stolen;
+  };
   const int home = blockIdx.x % a.np;
   __syncthreads();
 
@@ -248,9 +276,12 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
     if (held > 0) {
       // ---- pop up to 64 parents (newest first: depth-first order) -------
       const int n = held < kWave ? (int)held : kWave;
-      uint32_t nc = 0, meta = 0, slot = 0;
+      uint32_t nc = 0, meta = 0, slot = 0, dep = 0, lval = 0;
       if (lane < n) {
-        const uint32_t *r = rec_at<TREE>(deq, (top - 1 - lane) & dq_mask);
+        const long at = (top - 1 - lane) & dq_mask;
+        uint32_t r[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) r[w] = deq[w * dq_cap + at];
 #pragma unroll
         for (int w = 0; w < W; ++w) s_pst[lane * W + w] = r[w];
         meta = r[W - 1];
@@ -266,15 +297,43 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
             }
             nc = lo;
           }
-          atomicMax(&s_maxd[slot], depth);
+          dep = depth;
         } else {
           const int32_t ca = a.child_a[r[0]];
           nc = ca == 0 ? 0 : 2;
-          if (ca == 0) atomicAdd(&s_lsum[slot], (unsigned long long)a.value[r[0]]);
+          if (ca == 0) lval = a.value[r[0]];
         }
-        atomicAdd(&s_nodes[slot], 1ull);
-        if (nc == 0) atomicAdd(&s_leaves[slot], 1ull);
-        atomicAdd(&s_hold[slot], (int)nc - 1);
+      }
+      // per-tree accounting: one lane when the batch is all one tree
+      {
+        const uint32_t slot0 = __shfl(slot, 0, 64);
+        const bool active = lane < n;
+        if (__all(!active || slot == slot0)) {
+          const unsigned long long leafmask = __ballot(active && nc == 0);
+          uint32_t dmax = active ? dep : 0;
+          unsigned long long lsum = active ? lval : 0;
+          int dnc = active ? (int)nc - 1 : 0;
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t od = __shfl_xor(dmax, o, 64);
+            dmax = od > dmax ? od : dmax;
+            lsum += __shfl_xor(lsum, o, 64);
+            dnc += __shfl_xor(dnc, o, 64);
+          }
+          if (lane == 0) {
+            s_nodes[slot0] += (unsigned long long)n;
+            s_leaves[slot0] += (unsigned long long)__popcll(leafmask);
+            s_hold[slot0] += dnc;
+            if (TREE == 1 && dmax > s_maxd[slot0]) s_maxd[slot0] = dmax;
+            if (TREE == 0) s_lsum[slot0] += lsum;
+          }
+        } else if (active) {
+          atomicAdd(&s_nodes[slot], 1ull);
+          if (nc == 0) atomicAdd(&s_leaves[slot], 1ull);
+          atomicAdd(&s_hold[slot], (int)nc - 1);
+          if (TREE == 1) atomicMax(&s_maxd[slot], dep);
+          if (TREE == 0 && lval) atomicAdd(&s_lsum[slot], (unsigned long long)lval);
+        }
       }
       top -= n;
       uint32_t total;
@@ -299,24 +358,25 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
             if (s_off[mid] <= c) lo = mid; else hi = mid - 1;
           }
           const uint32_t *pr = &s_pst[lo * W];
-          uint32_t *dst = rec_at<TREE>(deq, (top + c) & dq_mask);
+          const long at = (top + c) & dq_mask;
           if (TREE == 1) {
             const uint32_t i = c - s_off[lo];
             uint32_t ch[5];
             const uint32_t par[5] = {pr[0], pr[1], pr[2], pr[3], pr[4]};
             uts_spawn(par, i, ch);
 #pragma unroll
-            for (int w = 0; w < 5; ++w) dst[w] = ch[w];
-            dst[5] = pr[5] + 1;  // depth + 1, same slot
+            for (int w = 0; w < 5; ++w) deq[w * dq_cap + at] = ch[w];
+            deq[5 * dq_cap + at] = pr[5] + 1;  // depth + 1, same slot
           } else {
             const uint32_t node = pr[0];
-            dst[0] = (c - s_off[lo]) == 0 ? (uint32_t)a.child_a[node] : (uint32_t)a.child_b[node];
-            dst[1] = pr[1];
+            deq[at] = (c - s_off[lo]) == 0 ? (uint32_t)a.child_a[node] : (uint32_t)a.child_b[node];
+            deq[dq_cap + at] = pr[1];
           }
         }
       }
       top += total;
       ++batches;
+      st_nodes += n;
       __syncthreads();
       // ---- donate the oldest half while others are hungry ----------------
       if ((batches & 3) == 0 && top - bot >= 2 * kWave) {
@@ -341,20 +401,23 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
           __syncthreads();
           const int k = s_k;
           if (k > 0) {
-            uint32_t *dstp = a.pool_rec + ((long)p * a.pc + s_base) * W;
+            uint32_t *pool = a.pool_rec + (long)p * a.pc * W;
             for (int i = lane; i < k; i += kWave) {
-              const uint32_t *r = rec_at<TREE>(deq, (bot + i) & dq_mask);
-              const uint32_t sl = meta_slot(r[W - 1]);
+              const long at = (bot + i) & dq_mask;
+              const uint32_t sl = meta_slot(deq[(W - 1) * dq_cap + at]);
 #pragma unroll
-              for (int w = 0; w < W; ++w) dstp[(long)i * W + w] = r[w];
+              for (int w = 0; w < W; ++w) pool[(long)w * a.pc + s_base + i] = deq[w * dq_cap + at];
               atomicAdd(&s_hold[sl], -1);
               atomicAdd(&s_cred[sl], -1);  // transferred to the pool (may go < 0)
             }
             bot += k;
+            ++st_don;
+            st_donr += k;
             __syncthreads();
-            // keep >= 1 credit for every slot still held; buy the rest
+            // keep >= 1 credit for every slot still held or with counts not
+            // yet flushed (settle flushes them under that credit); buy the rest
             for (int s = lane; s < kMaxSlots; s += kWave) {
-              const int want = s_hold[s] > 0 ? 1 : 0;
+              const int want = (s_hold[s] > 0 || s_nodes[s] != 0) ? 1 : 0;
               if (s_cred[s] < want) {
                 __hip_atomic_fetch_add(a.credit + (long)s * HS_LINE_WORDS,
                                        (uint32_t)(want - s_cred[s]), __ATOMIC_RELAXED, HS_AGENT);
@@ -372,7 +435,11 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
           __syncthreads();
         }
       }
-      if ((batches & 63) == 0 && g.aborted()) return;
+      if ((batches & 63) == 0 && g.aborted()) {
+        flush_stats();
+        return;
+      }
+      tick(true);
       continue;
     }
 
@@ -402,17 +469,24 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
     const int got = s_got;
     if (got > 0) {
       const int p = (int)s_pst[0];
-      const uint32_t *src = a.pool_rec + ((long)p * a.pc + s_pst[1]) * W;
+      const long from = s_pst[1];
+      const uint32_t *pool = a.pool_rec + (long)p * a.pc * W;
       __syncthreads();
       for (int i = lane; i < got; i += kWave) {
-        uint32_t *dst = rec_at<TREE>(deq, (top + i) & dq_mask);
+        const long at = (top + i) & dq_mask;
+        uint32_t last = 0;
 #pragma unroll
-        for (int w = 0; w < W; ++w) dst[w] = src[(long)i * W + w];
-        const uint32_t sl = meta_slot(dst[W - 1]);
+        for (int w = 0; w < W; ++w) {
+          last = pool[(long)w * a.pc + from + i];
+          deq[w * dq_cap + at] = last;
+        }
+        const uint32_t sl = meta_slot(last);
         atomicAdd(&s_hold[sl], 1);
         atomicAdd(&s_cred[sl], 1);  // pool records carry their credits
       }
       top += got;
+      ++st_steal;
+      st_stolen += got;
       __syncthreads();
       if (lane == 0) st_rel(a.pool_meta + (long)p * 64, 0);
       if (hungry && lane == 0) add_rlx(a.hungry, 0xffffffffu);
@@ -450,10 +524,10 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
         if (lane == 0 && TREE == 1) {
           uint32_t st[5];
           uts_root(s_pst[1], st);
-          uint32_t *dst = rec_at<TREE>(deq, top & dq_mask);
+          const long at = top & dq_mask;
 #pragma unroll
-          for (int w = 0; w < 5; ++w) dst[w] = st[w];
-          dst[W - 1] = slot << 16;
+          for (int w = 0; w < 5; ++w) deq[w * dq_cap + at] = st[w];
+          deq[(W - 1) * dq_cap + at] = slot << 16;
           s_hold[slot] = 1;
           s_cred[slot] = 1;
         }
@@ -480,7 +554,11 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
       s_exit = fin;
     }
     __syncthreads();
-    if (s_exit) return;
+    tick(false);
+    if (s_exit) {
+      flush_stats();
+      return;
+    }
   }
 }
 
@@ -572,7 +650,8 @@ struct Engine {
 
   // Puts one record (W words) into pool 0 with its credit, as a root.
   int seed_root(const uint32_t *rec, int slot) {
-    HS_HIP(hipMemcpy(a.pool_rec, rec, W * 4, hipMemcpyHostToDevice));
+    for (int w = 0; w < W; ++w)  // entry 0 of pool 0, word-major
+      HS_HIP(hipMemcpy(a.pool_rec + (size_t)w * a.pc, rec + w, 4, hipMemcpyHostToDevice));
     const uint32_t one = 1;
     HS_HIP(hipMemcpy(a.pool_meta + 32, &one, 4, hipMemcpyHostToDevice));
     HS_HIP(hipMemcpy(a.credit + (size_t)slot * HS_LINE_WORDS, &one, 4, hipMemcpyHostToDevice));
@@ -597,8 +676,45 @@ struct Engine {
   }
 };
 
+// HS_UTS_STATS=1 in the environment: per-WG statistics of the next run,
+// summarised on stderr (busy/idle time, batch fill, donations, steals).
+int stats_begin(Engine &e) {
+  const char *v = std::getenv("HS_UTS_STATS");
+  if (!v || !*v || *v == '0') return HS_OK;
+  int rc;
+  if ((rc = e.ar.alloc(&e.a.stats, (size_t)e.blocks * kStatWords))) return rc;
+  HS_HIP(hipMemset(e.a.stats, 0, (size_t)e.blocks * kStatWords * 8));
+  return HS_OK;
+}
+
+int stats_report(Engine &e, double kernel_ms) {
+  if (!e.a.stats) return HS_OK;
+  std::vector<unsigned long long> st((size_t)e.blocks * kStatWords);
+  HS_HIP(hipMemcpy(st.data(), e.a.stats, st.size() * 8, hipMemcpyDeviceToHost));
+  unsigned long long sum[kStatWords] = {};
+  std::vector<unsigned long long> nodes(e.blocks);
+  for (int b = 0; b < e.blocks; ++b) {
+    for (int k = 0; k < kStatWords; ++k) sum[k] += st[(size_t)b * kStatWords + k];
+    nodes[b] = st[(size_t)b * kStatWords + 3];
+  }
+  std::sort(nodes.begin(), nodes.end());
+  const double busy = (double)sum[0], idle = (double)sum[1];
+  std::fprintf(stderr,
+               "uts stats: %d WGs, %.3f ms: busy %.1f%% (%.3f ms/WG), batches %llu, "
+               "nodes/batch %.1f, donations %llu (%.1f rec), steals %llu (%.1f rec), "
+               "nodes/WG min %llu med %llu max %llu\n",
+               e.blocks, kernel_ms, 100.0 * busy / std::max(1.0, busy + idle),
+               busy / 1e5 / e.blocks, sum[2], (double)sum[3] / std::max(1ull, sum[2]), sum[4],
+               (double)sum[5] / std::max(1ull, sum[4]), sum[6],
+               (double)sum[7] / std::max(1ull, sum[6]), nodes.front(), nodes[nodes.size() / 2],
+               nodes.back());
+  return HS_OK;
+}
+
 template <int TREE>
 int run_single(hs_ctx *c, Engine &e, hs_uts_result *res) {
+  int src = stats_begin(e);
+  if (src) return src;
   hipEvent_t e0, e1;
   HS_HIP(hipEventCreate(&e0));
   HS_HIP(hipEventCreate(&e1));
@@ -613,6 +729,7 @@ int run_single(hs_ctx *c, Engine &e, hs_uts_result *res) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   res->kernel_ms = ms;
+  if ((src = stats_report(e, ms))) return src;
   return e.read_slot(0, res);
 }
 

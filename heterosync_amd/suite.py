@@ -1,0 +1,180 @@
+"""Measurement suite for the synchronization hot path on one GPU.
+
+  C1  plain faMutex 32 WGs x 64 lanes, numLdSt=10, numCSIters=100 (10 launches)
+  C2  grid barriers 1024 WGs x 256 lanes, numLdSt=10, numCSIters=1000
+  C3  64-instance mutex/semaphore mix: batched persistent launch vs plain
+  C4  relaxed atomics 1024 WGs x 256 lanes x 10^4 iterations
+  C5  UTS serving loop (Poisson arrivals, geometric SHA-1 trees) vs one
+      launch per tree
+
+Every measurement is parity-checked (oracle golden / closed form / exact
+counts) and reported next to its unit rate.  Usage:
+  python -m heterosync_amd.suite [c1 c2 c3 c4 c5]
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import sys
+import time
+
+import numpy as np
+
+import heterosync_amd as hs
+from heterosync_amd.mix import make_mix
+
+
+def _oracle():
+    from tests.conftest import OracleHelpers, _load_oracle
+    o = _load_oracle()
+    return o, OracleHelpers(o)
+
+
+def c1(dev, oh):
+    s, r = dev.syncprim("faMutex", 10, 32, 100, reps=10, timeout_s=30.0)
+    ok = bool(np.array_equal(s, oh.golden(7, 10, 32, 100, 10, 64, dev.info.num_cu, 0)))
+    return {"acq_rel_per_s": r.cs_entries / (r.roi_ms / 1e3), "roi_ms": r.roi_ms,
+            "acquires": r.cs_entries, "parity_ok": ok and r.state_ok == 1}
+
+
+def c2(dev, oh, scopes=(0, 1)):
+    out = {}
+    ncu = dev.info.num_cu
+    for name in ("atomicTreeBarrUniq", "atomicTreeBarrUniqLocalExch", "lfTreeBarrUniq",
+                 "lfTreeBarrUniqLocalExch"):
+        pid = hs.prim_id(name)
+        for sc in scopes:
+            s, r = dev.syncprim(pid, 10, 4 * ncu, 1000, wg_size=256, reps=10, timeout_s=30.0,
+                                sync_scope=sc)
+            ok = bool(np.array_equal(s, oh.golden(pid, 10, 4 * ncu, 1000, 10, 256, ncu, 0)))
+            out[f"{name}_{'strict' if sc else 'xcd'}"] = {
+                "episodes_per_s": r.episodes / (r.roi_ms / 1e3),
+                "parity_ok": ok and r.state_ok == 1}
+    return out
+
+
+def c3(dev, oh, seed=3, reps=1, tune=True):
+    """One launch (reps=1) of the seeded 64-instance mix, batched vs plain."""
+    mix = make_mix(seed=seed, n=64)
+    ncu = dev.info.num_cu
+    kw = {}
+    tuned = None
+    if tune:
+        wgs, poll, bmax, best = dev.batch_autotune(mix, reps=reps)
+        kw = dict(persistent_wgs=wgs, poll_sleep=poll, backoff=(0, bmax))
+        tuned = {"persistent_wgs": wgs, "poll_sleep": poll, "backoff_max": bmax,
+                 "calibration_ms": best}
+    st = [np.arange(hs.storage_words(p, nw, ncu, 64, L), dtype=np.int32) for p, nw, it, L in mix]
+    entries, rb = dev.batch(mix, st, reps=reps, use_graph=1, timeout_s=60.0, **kw)
+    ok = rb.all_ok == 1
+    for (p, nw, it, L), s in zip(mix, st):
+        ok = ok and bool(np.array_equal(s, oh.golden(p, L, nw, it, reps, 64, ncu, 0)))
+    plain_ms = 0.0
+    plain_acq = 0
+    plain_ok = True
+    for (p, nw, it, L), s in zip(mix, st):
+        ps, r = dev.syncprim(p, L, nw, it, reps=reps, timeout_s=60.0)
+        plain_ms += r.roi_ms
+        plain_acq += r.cs_entries + r.rd_entries
+        plain_ok = plain_ok and bool(np.array_equal(ps, s)) and r.state_ok == 1
+    batched = rb.acquires / (rb.roi_ms / 1e3)
+    plain = plain_acq / (plain_ms / 1e3)
+    return {"instances": len(mix), "acquires": int(rb.acquires),
+            "batched_acq_rel_per_s": batched, "batched_roi_ms": rb.roi_ms,
+            "plain_acq_rel_per_s": plain, "plain_roi_ms_sum": plain_ms,
+            "speedup": batched / plain, "tuned": tuned,
+            "parity_ok": ok and plain_ok and plain_acq == rb.acquires}
+
+
+def c4(dev, oh, iters=10000):
+    o = oh.lib
+    ncu = dev.info.num_cu
+    nw = 4 * ncu
+    out = {}
+    # seqlock: 1 writer WG per 64 readers (writer period 65), 16 locks
+    seq, outa, torn, r = dev.seqlock(nw, 256, 16, 65, iters, 0)
+    want = np.zeros(len(seq), np.uint32)
+    o.hso_seqlock_expected(nw, 16, 65, iters, want.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                           len(seq))
+    out["seqlock"] = {"ms": r.kernel_ms, "atomic_ops": r.atomic_ops,
+                      "atomic_ops_per_s": r.atomic_ops / (r.kernel_ms / 1e3),
+                      "passes_per_s": nw * iters / (r.kernel_ms / 1e3),
+                      "parity_ok": bool(np.array_equal(seq, want)) and torn == 0}
+    # flags
+    rnd = np.zeros(nw * iters, np.float32)
+    o.hso_flags_rand(rnd.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), rnd.size, 2018)
+    wd = o.hso_flags_dirty(rnd.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), nw, iters,
+                           ctypes.c_float(0.33))
+    fo, dirty, r = dev.flags(nw, 256, 16, 4, 0.33, iters, rnd)
+    out["flags"] = {"ms": r.kernel_ms, "dirty": dirty,
+                    "worker_iters_per_s": (nw - 1) * iters / (r.kernel_ms / 1e3),
+                    "parity_ok": dirty == wd}
+    # refCounter: 32 sharers, 10^4 inc/dec pairs, spacing 8
+    c0, c1_, d0, d1, r = dev.refcounter(32, nw, 256, iters, 8)
+    groups = nw // 32
+    okr = not c0.any() and not c1_.any() and bool(
+        np.all(d0.reshape(32, groups, 256).max(axis=0) == 1))
+    out["refCounter"] = {"ms": r.kernel_ms, "atomic_ops": r.atomic_ops,
+                         "atomic_ops_per_s": r.atomic_ops / (r.kernel_ms / 1e3),
+                         "parity_ok": okr}
+    # split counter
+    cnt, so, r = dev.splitcounter(nw, 256, iters)
+    out["splitCounter"] = {"ms": r.kernel_ms, "atomic_ops": r.atomic_ops,
+                           "adds_per_s": cnt.size * iters / (r.kernel_ms / 1e3),
+                           "parity_ok": bool(np.all(cnt == iters)) and
+                           int(so.max()) <= cnt.size * iters}
+    return out
+
+
+def c5(dev, oh, n=32, seed=9, load=0.8, slots=16):
+    """Poisson arrivals of geometric (b0=4) trees, depth ~ U[6, 14]."""
+    rng = np.random.default_rng(seed)
+    trees = [(int(rng.integers(0, 1 << 30)), int(rng.integers(6, 15))) for _ in range(n)]
+    # plain: one launch per tree, back to back (also gives the capacity)
+    plain = []
+    t0 = time.perf_counter()
+    for sd, d in trees:
+        r = dev.uts_geo(sd, 4.0, d)
+        plain.append((r.nodes, r.leaves, r.max_depth, r.kernel_ms))
+    plain_wall = time.perf_counter() - t0
+    total_nodes = sum(p[0] for p in plain)
+    capacity = total_nodes / sum(p[3] / 1e3 for p in plain)  # nodes/s of the engine
+    mean_nodes = total_nodes / n
+    lam = load * capacity / mean_nodes  # trees per second
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
+    arrivals -= arrivals[0]
+    reqs = [(sd, d, 4.0, float(a)) for (sd, d), a in zip(trees, arrivals)]
+    resp, summ = dev.uts_serve(reqs, slots=slots, timeout_s=120.0)
+    ok = all((r.nodes, r.leaves, r.max_depth) == p[:3] for r, p in zip(resp, plain))
+    # plain under the same arrivals: FIFO, one tree at a time
+    t = 0.0
+    lat = []
+    for a, p in zip(arrivals, plain):
+        t = max(t, a) + p[3] / 1e3
+        lat.append(t - a)
+    lat = np.sort(lat)
+    return {"trees": n, "total_nodes": int(total_nodes), "offered_load": load,
+            "serving": {"nodes_per_s": summ.total_nodes / summ.wall_s, "wall_s": summ.wall_s,
+                        "p50_s": summ.p50_s, "p99_s": summ.p99_s},
+            "plain_one_launch_per_tree": {
+                "nodes_per_s": total_nodes / (sum(p[3] for p in plain) / 1e3),
+                "wall_s_back_to_back": plain_wall,
+                "p50_s": float(lat[int(0.5 * (n - 1) + 0.5)]),
+                "p99_s": float(lat[int(0.99 * (n - 1) + 0.5)])},
+            "parity_ok": ok}
+
+
+def main(argv):
+    which = argv or ["c1", "c2", "c3", "c4", "c5"]
+    o, oh = _oracle()
+    dev = hs.Device(0)
+    out = {}
+    for w in which:
+        out[w] = globals()[w](dev, oh)
+        print(json.dumps({w: out[w]}), flush=True)
+    dev.close()
+    return out
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -183,7 +183,6 @@ int hso_region_passes(int p, int num_wgs, int iters, int reps, int num_cu,
   if (m < 0 || m != nregions) return -1;
   for (int r = 0; r < m; ++r) passes[r] = 0;
   if (is_barrier(p)) {
-    if (p >= 37) return -1; /* SRB variants: not built */
     if (num_wgs > num_cu && num_wgs % num_cu) return -1; /* RD:80 */
     for (int r = 0; r < m; ++r) passes[r] = ir * ((is_lx(p) && perCU > 1) ? 2 : 1);
   } else if (is_mutex(p)) {

@@ -35,9 +35,9 @@ def test_prim_table_and_sizes(oracle):
         assert hs.prim_name(pid) == name
     assert hs.prim_id("bogus") == -1
     lib = hs.lib()
-    assert [p for p in range(47) if lib.hs_prim_supported(p)] == \
-        list(range(16)) + list(range(20, 32))
-    for pid in list(range(16)) + list(range(20, 32)):
+    supported = list(range(16)) + list(range(20, 32)) + list(range(37, 43))
+    assert [p for p in range(47) if lib.hs_prim_supported(p)] == supported
+    for pid in supported:
         for nw, ncu, wg, L in [(32, 256, 64, 10), (1024, 256, 256, 10), (3, 4, 64, 0),
                                (512, 256, 64, 100)]:
             assert hs.storage_words(pid, nw, ncu, wg, L) == \

@@ -14,6 +14,13 @@ UMUTEX = [20, 21, 22, 23]
 SEM = list(range(8, 16))
 USEM = list(range(24, 32))
 BARRIER = [0, 1, 2, 3]
+SRB = [37, 38, 39, 40, 41, 42]
+
+
+def srb_scope(prim, nw, ncu, scope):
+    """The XCD-aware form is used only where the SRB barrier is a tree."""
+    per = -(-nw // min(nw, ncu))
+    return scope if prim in (37, 38) and per > 4 else 1
 
 
 def check(dev, ohelp, prim, L, nw, iters, reps=10, wg=64, mode=0, literal=False, **kw):
@@ -75,6 +82,27 @@ def test_barrier_full_config(dev, ohelp, prim, scope):
     ncu = dev.info.num_cu
     r = check(dev, ohelp, prim, 10, 4 * ncu, 1000, wg=256, mode=1, sync_scope=scope)
     assert r.episodes == 10000 and r.sync_scope_used == scope
+
+
+@pytest.mark.parametrize("prim", SRB)
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("scope", [0, 1])
+def test_srb_small(dev, ohelp, prim, mode, scope):
+    """Sense-reversing family: flat below 5 WGs per group, tree above."""
+    ncu = dev.info.num_cu
+    for nw in (1, 5, 64, ncu, 2 * ncu, 4 * ncu, 8 * ncu):
+        r = check(dev, ohelp, prim, 3, nw, 12, reps=3, mode=mode, sync_scope=scope)
+        assert r.sync_scope_used == srb_scope(prim, nw, ncu, scope)
+    check(dev, ohelp, prim, 2, 2 * ncu, 3, reps=2, mode=mode, wg=128, sync_scope=scope)
+    check(dev, ohelp, prim, 2, 3, 4, reps=2, mode=mode, literal=True, sync_scope=scope)
+
+
+@pytest.mark.parametrize("prim", SRB)
+def test_srb_full_config(dev, ohelp, prim):
+    """1024 WGs x 256 threads, L=10, 1000 iterations x 2 launches."""
+    ncu = dev.info.num_cu
+    r = check(dev, ohelp, prim, 10, 4 * ncu, 1000, reps=2, wg=256, mode=1)
+    assert r.episodes == 2000
 
 
 def test_fa_mutex_c1(dev, ohelp):
