@@ -149,7 +149,7 @@ const char *hs_prim_name(int id) {
 }
 
 int hs_prim_supported(int id) {
-  return (id >= 0 && id <= 15) || (id >= 20 && id <= 31) || (id >= 37 && id <= 42);
+  return (id >= 0 && id <= 35) || (id >= 37 && id <= 42);
 }
 
 long hs_storage_words(int p, int num_wgs, int num_cu, int wg, int L) {

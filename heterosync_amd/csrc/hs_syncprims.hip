@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(1024) k_mutex(SPArgs a) {
   }
 }
 
-template <bool EBO, bool UNIQ, int MODE>
+template <bool EBO, bool UNIQ, int MODE, bool PRIO = false>
 __global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
   __shared__ int s_ok;
   __shared__ int32_t sink[1024];
@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
   __syncthreads();
   if (!s_ok) return;
   for (int i = 0; i < a.iters; ++i) {
-    if (is_master()) s_ok = sem_wait<EBO>(lk, writer, a.bo, g);
+    if (is_master()) s_ok = sem_wait<EBO, PRIO>(lk, writer, a.bo, g);
     __syncthreads();
     if (!s_ok) return;
     if (writer) {
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
         st_rlx(lk + kLkCs, ld_rlx(lk + kLkCs) + 1);  // exclusive
       else
         add_rlx(lk + kLkRd, 1);
-      s_ok = sem_post(lk, writer, a.bo, g);
+      s_ok = sem_post<PRIO>(lk, writer, a.bo, g);
     }
     __syncthreads();
     if (!s_ok) return;
@@ -265,7 +265,13 @@ __global__ void k_init_locks(uint32_t *locks, int nlocks, uint32_t sem_max,
 
 bool is_barrier(int p) { return p < 4 || (p >= 37 && p <= 42); }
 bool is_mutex(int p) { return (p >= 4 && p <= 7) || (p >= 20 && p <= 23); }
-bool is_sem(int p) { return (p >= 8 && p <= 15) || (p >= 24 && p <= 31); }
+bool is_sem(int p) { return p >= 8 && p <= 35; }
+// per-group locks and regions: the uniq mutexes and uniq semaphores.  The
+// priority EBO semaphores 32-35 are dispatched to the global kernel
+// (main.hip.cpp:2455-2470 -> invokeEBOSemaphorePriority) although their
+// storage is sized like the uniq semaphores' (main.hip.cpp:2109-2128).
+bool is_uniq(int p) { return p >= 20 && p <= 31; }
+bool is_prio(int p) { return (p >= 16 && p <= 19) || (p >= 32 && p <= 35); }
 
 using KernelFn = void (*)(SPArgs);
 
@@ -296,13 +302,14 @@ KernelFn pick_kernel(int p) {
   if (p >= 12 && p <= 15) return k_sem<true, false, MODE>;
   if (p >= 24 && p <= 27) return k_sem<false, true, MODE>;
   if (p >= 28 && p <= 31) return k_sem<true, true, MODE>;
+  if (p >= 16 && p <= 19) return k_sem<false, false, MODE, true>;
+  if (p >= 32 && p <= 35) return k_sem<true, false, MODE, true>;
   return nullptr;
 }
 
 uint32_t sem_count_of(int p) {
   static const uint32_t kN[4] = {1, 2, 10, 120};  // spinSem1/2/10/120
-  if (p >= 8 && p <= 15) return kN[(p - 8) % 4];
-  if (p >= 24 && p <= 31) return kN[(p - 24) % 4];
+  if (p >= 8 && p <= 35 && !(p >= 20 && p <= 23)) return kN[(p - 8) % 4];
   return 0;
 }
 
@@ -357,7 +364,7 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   const int ncu = c->num_cu;
   const int nab = std::min(nw, ncu);
   const int per = (nw + nab - 1) / nab;
-  const bool uniq = p >= 20;
+  const bool uniq = is_uniq(p);
   if (is_barrier(p) && nw > ncu && nw % ncu) return HS_EINVAL;  // RD:80
   if (is_sem(p) && uniq && nw % nab) return HS_EINVAL;  // each group needs its writer
   if (n != hs_storage_words(p, nw, ncu, wg, L)) return HS_EINVAL;
@@ -532,6 +539,7 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
       }
       if (is_sem(p) && (!uniq || l < nab))  // count back to max, no writer waiting
         ok = ok && lk[kLkHead] == sem_max;
+      if (is_prio(p)) ok = ok && lk[kLkPrio] == 0;  // no post in flight
     }
     const uint32_t *lk0 = &locks[0];
     const bool semw = is_sem(p);

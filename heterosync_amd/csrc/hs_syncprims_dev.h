@@ -20,11 +20,15 @@ constexpr int kWordsPerLine = 16;  // oracle layout constant (HM:12)
 //             semaphore state: count | writerWaiting << 16
 //   word  3 : semaphore max count
 //   word 32 : tail / turn                 (line 1, polled by FA)
+//   word 48 : posts in flight             (line 1, priority semaphores)
 //   word 64 : writer / CS entry counter   (line 2, plain, under the lock)
 //   word 96 : reader entry counter        (line 3, atomic)
 constexpr int kLockWords = 128;
 constexpr int kLkHead = 0, kLkMax = 3;
 constexpr int kLkTail = 32, kLkCs = 64, kLkRd = 96;
+// priority semaphores: number of posts in flight (HSS:26, 591-595), on the
+// tail's line, which semaphores do not otherwise use
+constexpr int kLkPrio = 48;
 
 // Barrier block: 64 words per level instance (count @0, generation @32).
 constexpr int kBarWords = 64;
@@ -241,11 +245,26 @@ __device__ __forceinline__ int sem_try_wait(uint32_t *lk, bool writer, Guard &g)
   }
 }
 
-template <bool EBO>
+// Priority variants (HSS:430-470): an entry attempt first waits until no
+// post is in flight, backing off exponentially while one is.
+__device__ __forceinline__ bool sem_yield_to_posts(const uint32_t *lk, const Backoff &bo,
+                                                   Guard &g) {
+  uint32_t b = bo.min_q;
+  while (ld_rlx(lk + kLkPrio) != 0) {
+    if (!g.ok()) return false;
+    sleep_quanta(b);
+    b = ((b << 1) + 1) & bo.max_q;
+    if (b < bo.min_q) b = bo.min_q;
+  }
+  return true;
+}
+
+template <bool EBO, bool PRIO = false>
 __device__ __forceinline__ bool sem_wait(uint32_t *lk, bool writer,
                                          const Backoff &bo, Guard &g) {
   uint32_t b = bo.min_q;
   for (;;) {
+    if (PRIO && !sem_yield_to_posts(lk, bo, g)) return false;
     const int r = sem_try_wait(lk, writer, g);
     if (r > 0) return true;
     if (r < 0) return false;
@@ -262,10 +281,15 @@ __device__ __forceinline__ bool sem_wait(uint32_t *lk, bool writer,
   }
 }
 
-// Post: one release RMW (also publishes a writer's CS data).
+// Post: one release RMW (also publishes a writer's CS data).  The priority
+// form brackets it with the in-flight count that entering WGs yield to
+// (HSS:591-595, 639-640).
+template <bool PRIO = false>
 __device__ __forceinline__ bool sem_post(uint32_t *lk, bool writer,
                                          const Backoff &, Guard &) {
+  if (PRIO) add_rlx(lk + kLkPrio, 1u);
   add_rel(lk + kLkHead, writer ? ld_rlx(lk + kLkMax) : 1u);
+  if (PRIO) add_rlx(lk + kLkPrio, 0xffffffffu);
   return true;
 }
 

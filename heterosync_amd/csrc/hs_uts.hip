@@ -17,18 +17,23 @@
 //     counts across the wave and then generates the children 64 at a time,
 //     one SHA-1 compression per lane per round (no divergence from uneven
 //     fan-out);
-//   * load balance: WGs donate the oldest half of their deque to shared
-//     pools while other WGs are hungry and take from pools when empty;
+//   * load balance by direct hand-off: an idle WG appends its id to a
+//     device-wide request ring; a busy WG with at least two batches of
+//     work claims the oldest request and writes the oldest half of its
+//     deque into that WG's private inbox, then releases the record count
+//     on the receiver's own line.  Receivers poll only their own line and
+//     every request is served exactly once, by one donor;
 //   * termination and per-tree completion: credit counting.  Each tree has
-//     a device counter; nodes in pools are always credited; a WG holding
-//     nodes of a tree keeps at least one credit for it and returns its
-//     credits when it holds no more, so the counter reaches zero exactly
-//     when the tree is finished, and the WG that takes it to zero knows.
+//     a device counter; records in flight to an inbox carry one credit
+//     each; a WG holding nodes of a tree keeps at least one credit for it
+//     and returns its credits when it holds no more, so the counter reaches
+//     zero exactly when the tree is finished, and the WG that takes it to
+//     zero knows.
 //
 // The serving loop (hs_uts_serve) runs the same engine persistently over
 // many trees ("slots"): requests are admitted from a host queue as slots
-// and pool memory free up, and finished trees are reported back while
-// other trees are still running.
+// free up, and finished trees are reported back while other trees are
+// still running.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -48,6 +53,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kMaxSlots = 64;
 constexpr int kMaxThr = 128;
+constexpr long kGrow = 1024;  // deque size below which a WG pops oldest first
 
 // ---------------------------------------------------------------------------
 // SHA-1 (FIPS 180-4) of one padded 64-byte block.
@@ -125,10 +131,19 @@ struct UtsArgs {
   // deques: num_blocks rings of dq_cap records
   uint32_t *deq;
   int dq_log2;
-  // pools
-  uint32_t *pool_rec;   // np * pc records
-  uint32_t *pool_meta;  // np * 64 words: lock @0, count @32
-  int np, pc;
+  // work requests: nrq rings; ring r has its head and tail in one 64-bit
+  // word at rq[r * 32] (head low) and rq_mask + 1 entries at
+  // rq_ring[r * (rq_mask + 1)] holding requester ids + 1 (0 = not yet
+  // written).  WG b requests on ring b % nrq; donors visit the rings in turn.
+  uint32_t *rq;
+  uint32_t *rq_ring;
+  uint32_t rq_mask;
+  int nrq;
+  // direct hand-off: per WG a record count on its own line and an inbox of
+  // wbox_cap records (blocked by 64 like the deques)
+  uint32_t *wbox_n;
+  uint32_t *wbox;
+  int wbox_cap;
   // per-slot device state (kMaxSlots each, 64-bit counters)
   uint32_t *credit;     // kMaxSlots * 32 words (own line each)
   unsigned long long *nodes, *leaves, *leafsum;
@@ -141,8 +156,7 @@ struct UtsArgs {
   const uint32_t *value;
   const int32_t *child_a, *child_b;
   // control
-  uint32_t *hungry;  // idle WG count
-  uint32_t *ctl;     // [0] abort, [32] pool overflow
+  uint32_t *ctl;     // [0] abort, [32] deque overflow
   uint64_t ticks;
   // serving (null when not serving)
   uint32_t *live;          // number of live (admitted, unfinished) trees
@@ -153,11 +167,12 @@ struct UtsArgs {
   uint32_t *resp;          // per slot: 8 words (nodes lo/hi, leaves lo/hi, maxd, done seq)
   const uint32_t *stop;
   // optional per-WG statistics (null: off), kStatWords each:
-  // busy ticks, idle ticks, batches, nodes, donations, donated, steals, stolen
+  // busy ticks, idle ticks, batches, nodes, donations, donated, steals,
+  // stolen, pop ticks, generate ticks, donate ticks, generate rounds
   unsigned long long *stats;
 };
 
-constexpr int kStatWords = 8;
+constexpr int kStatWords = 12;
 
 __device__ __forceinline__ uint32_t meta_slot(uint32_t m) { return m >> 16; }
 __device__ __forceinline__ uint32_t meta_depth(uint32_t m) { return m & 0xffffu; }
@@ -175,18 +190,33 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t *total) 
   return x - v;
 }
 
-// Pool lock (master lane).
-__device__ __forceinline__ bool pool_lock(uint32_t *lk, Guard &g) {
-  for (;;) {
-    if (ld_rlx(lk) == 0 && cas_acq(lk, 0, 1) == 0) return true;
-    if (!g.ok()) return false;
-    sleep_quanta(2);
+// Adds d to the held-node and credit counts of each valid lane's slot;
+// one LDS update when the whole wave holds records of a single slot.
+__device__ __forceinline__ void slot_count_add(int *hold, int *cred, uint32_t sl, bool valid,
+                                               int d) {
+  const uint32_t sl0 = __shfl(sl, 0, 64);
+  const unsigned long long vm = __ballot(valid);
+  if (__all(!valid || sl == sl0)) {
+    if ((threadIdx.x & 63) == 0 && vm) {
+      const int c = __popcll(vm) * d;
+      hold[sl0] += c;
+      cred[sl0] += c;
+    }
+  } else if (valid) {
+    atomicAdd(&hold[sl], d);
+    atomicAdd(&cred[sl], d);
   }
 }
 
-// Records are stored structure-of-arrays: word w of entry i of a deque or
-// pool with capacity `cap` is base[w * cap + i], so lane-parallel record
-// accesses are contiguous.
+// Deques are blocked by 64 records: the W words of record i sit in block
+// i / 64, word-major inside the block, so a batch of 64 consecutive records
+// is W contiguous 256-byte runs and a WG's hot region is a few KB.  Each
+// WG's deque is offset by one extra line from the previous one.
+constexpr long kDqSkew = 32;
+template <int W>
+__device__ __forceinline__ long dq_idx(long at, int w) {
+  return ((at >> 6) * W + w) * 64 + (at & 63);
+}
 
 template <int TREE, bool SERVE>
 __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
@@ -198,9 +228,10 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
   __shared__ unsigned long long s_nodes[kMaxSlots], s_leaves[kMaxSlots], s_lsum[kMaxSlots];
   __shared__ uint32_t s_maxd[kMaxSlots];
   __shared__ uint32_t s_thr[kMaxThr];
+  __shared__ uint32_t s_gmx[kMaxSlots];   // depth limit of each slot's tree
   const int lane = threadIdx.x;
   const long dq_cap = 1L << a.dq_log2, dq_mask = dq_cap - 1;
-  uint32_t *deq = a.deq + (long)blockIdx.x * dq_cap * W;
+  uint32_t *deq = a.deq + (long)blockIdx.x * (dq_cap * W + kDqSkew);
   Guard g;
   g.init(a.ctl, a.ticks);
   for (int s = lane; s < kMaxSlots; s += kWave) {
@@ -210,11 +241,28 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
     s_maxd[s] = 0;
   }
   for (int k = lane; k < kMaxThr; k += kWave) s_thr[k] = k < a.nthr ? a.thr[k] : 0xffffffffu;
+  // a slot's limit is written before its root exists; a WG gets records of a
+  // slot only by admitting the tree or through a delivery (acquire), and
+  // refreshes this copy on both
+  for (int k = lane; k < kMaxSlots; k += kWave) s_gmx[k] = ld_rlx(a.slot_gen_mx + k);
   long top = 0, bot = 0;  // deque holds [bot, top)
-  bool hungry = false;
+  bool requested = false;  // this WG's id is in the request ring
   uint32_t batches = 0;
+  uint32_t idle_q = 1;  // idle poll interval (sleep quanta), doubles to 32
+  uint32_t idle_iter = 0, donate_checks = 0;
+  const int me = blockIdx.x;
+  uint32_t *my_n = a.wbox_n + (long)me * HS_LINE_WORDS;
   unsigned long long st_busy = 0, st_idle = 0, st_nodes = 0, st_don = 0, st_donr = 0,
-                     st_steal = 0, st_stolen = 0;
+                     st_steal = 0, st_stolen = 0, st_tpop = 0, st_tgen = 0, st_tdon = 0,
+                     st_rounds = 0;
+  // phase clock: adds the time since the last call to `acc`
+  unsigned long long t_ph = 0;
+  auto phase = [&](unsigned long long &acc) {
+    if (!a.stats) return;
+    const unsigned long long now = wall_clock64();
+    acc += now - t_ph;
+    t_ph = now;
+  };
   unsigned long long t_mark = a.stats ? wall_clock64() : 0;
   // folds the time since the last mark into busy or idle
   auto tick = [&](bool busy) {
@@ -234,8 +282,11 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
     o[5] = st_donr;
     o[6] = st_steal;
     o[7] = st_stolen;
+    o[8] = st_tpop;
+    o[9] = st_tgen;
+    o[10] = st_tdon;
+    o[11] = st_rounds;
   };
-  const int home = blockIdx.x % a.np;
   __syncthreads();
 
   // Returns the slot credits / counts of every slot this WG no longer
@@ -271,17 +322,121 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
     __syncthreads();
   };
 
+  // Takes the k records a donor (or the host, for a root) left in this WG's
+  // inbox; their credits come with them.  Whole wave.
+  auto absorb = [&](int k) -> bool {
+    if (top - bot + k > dq_cap) {  // no room: report a deque overflow
+      if (lane == 0) {
+        st_rlx(a.ctl + 32, 1);
+        st_rlx(a.ctl, 1);
+      }
+      return false;
+    }
+    fence_acq();  // the donor released the records with the count
+    if (SERVE)
+      for (int s2 = lane; s2 < kMaxSlots; s2 += kWave) s_gmx[s2] = ld_rlx(a.slot_gen_mx + s2);
+    const uint32_t *box = a.wbox + (long)me * a.wbox_cap * W;
+    for (int i0 = 0; i0 < k; i0 += kWave) {
+      const int i = i0 + lane;
+      uint32_t sl = 0;
+      if (i < k) {
+        const long at = (top + i) & dq_mask;
+        uint32_t v[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) v[w] = box[dq_idx<W>(i, w)];
+#pragma unroll
+        for (int w = 0; w < W; ++w) deq[dq_idx<W>(at, w)] = v[w];
+        sl = meta_slot(v[W - 1]);
+      }
+      slot_count_add(s_hold, s_cred, sl, i < k, 1);
+    }
+    top += k;
+    ++st_steal;
+    st_stolen += k;
+    idle_q = 1;
+    __syncthreads();
+    if (lane == 0) st_rlx(my_n, 0);  // ordered before the next request (release)
+    requested = false;
+    return true;
+  };
+
+  // Claims the oldest work request, if any, and hands that WG the oldest
+  // half of this deque (at most one inbox).  Whole wave.
+  auto donate = [&]() {
+    int v = -1;
+    if (lane == 0) {
+      const int r = (me + (int)(donate_checks++)) & (a.nrq - 1);
+      uint32_t *ctl = a.rq + (long)r * HS_LINE_WORDS;
+      const uint64_t ht = __hip_atomic_load(reinterpret_cast<uint64_t *>(ctl),
+                                            __ATOMIC_RELAXED, HS_AGENT);
+      const uint32_t h = (uint32_t)ht, t = (uint32_t)(ht >> 32);
+      if (h != t && cas_rlx(ctl, h, h + 1) == h) {
+        uint32_t *ent = a.rq_ring + (long)r * (a.rq_mask + 1) + (h & a.rq_mask);
+        uint32_t id;
+        while ((id = ld_rlx(ent)) == 0 && g.ok()) {
+        }  // the requester writes its id right after taking the slot
+        if (id != 0) {
+          st_rlx(ent, 0);
+          v = (int)id - 1;
+        }
+      }
+    }
+    v = __shfl(v, 0, 64);
+    if (v < 0) return;
+    const int k = (int)std::min<long>((top - bot) / 2, (long)a.wbox_cap);
+    uint32_t *box = a.wbox + (long)v * a.wbox_cap * W;
+    for (int i0 = 0; i0 < k; i0 += kWave) {
+      const int i = i0 + lane;
+      uint32_t sl = 0;
+      if (i < k) {
+        const long at = (bot + i) & dq_mask;
+        uint32_t r[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) r[w] = deq[dq_idx<W>(at, w)];
+#pragma unroll
+        for (int w = 0; w < W; ++w) box[dq_idx<W>(i, w)] = r[w];
+        sl = meta_slot(r[W - 1]);
+      }
+      // each record leaves with one credit (s_cred may go < 0 here)
+      slot_count_add(s_hold, s_cred, sl, i < k, -1);
+    }
+    bot += k;
+    ++st_don;
+    st_donr += k;
+    __syncthreads();
+    // keep >= 1 credit for every slot still held or with counts not yet
+    // flushed (settle flushes them under that credit); buy the rest
+    for (int s = lane; s < kMaxSlots; s += kWave) {
+      const int want = (s_hold[s] > 0 || s_nodes[s] != 0) ? 1 : 0;
+      if (s_cred[s] < want) {
+        __hip_atomic_fetch_add(a.credit + (long)s * HS_LINE_WORDS, (uint32_t)(want - s_cred[s]),
+                               __ATOMIC_RELAXED, HS_AGENT);
+        s_cred[s] = want;
+      }
+    }
+    __syncthreads();
+    if (lane == 0) st_rel(a.wbox_n + (long)v * HS_LINE_WORDS, (uint32_t)k);
+  };
+
   for (;;) {
     const long held = top - bot;
     if (held > 0) {
-      // ---- pop up to 64 parents (newest first: depth-first order) -------
+      if (a.stats) t_ph = wall_clock64();
+      // a delivery for a request still outstanding, read now, used after
+      const uint32_t box_n = (lane == 0 && requested) ? ld_rlx(my_n) : 0;
+      // ---- pop up to 64 parents ------------------------------------------
+      // Newest first (depth-first, bounded memory) once the deque holds a
+      // donatable frontier; oldest first (breadth-first) while it is
+      // smaller, so that it grows into one: the frontier is what donations
+      // are cut from.
       const int n = held < kWave ? (int)held : kWave;
+      const bool bfs = held < kGrow;
       uint32_t nc = 0, meta = 0, slot = 0, dep = 0, lval = 0;
       if (lane < n) {
-        const long at = (top - 1 - lane) & dq_mask;
+        const long at = (bfs ? bot + lane : top - 1 - lane) & dq_mask;
         uint32_t r[W];
 #pragma unroll
-        for (int w = 0; w < W; ++w) r[w] = deq[w * dq_cap + at];
+        for (int w = 0; w < W; ++w) r[w] = deq[dq_idx<W>(at, w)];
 #pragma unroll
         for (int w = 0; w < W; ++w) s_pst[lane * W + w] = r[w];
         meta = r[W - 1];
@@ -289,7 +444,7 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
         if (TREE == 1) {
           const uint32_t depth = meta_depth(meta);
           const uint32_t h = r[4] & 0x7fffffffu;
-          if (depth < a.slot_gen_mx[slot] || depth == 0) {
+          if (depth < s_gmx[slot] || depth == 0) {
             int lo = 0, hi = a.nthr;  // count thresholds <= h
             while (lo < hi) {
               const int mid = (lo + hi) >> 1;
@@ -335,7 +490,7 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
           if (TREE == 0 && lval) atomicAdd(&s_lsum[slot], (unsigned long long)lval);
         }
       }
-      top -= n;
+      if (bfs) bot += n; else top -= n;
       uint32_t total;
       const uint32_t off = wave_excl_scan(lane < n ? nc : 0, &total);
       s_off[lane] = off;
@@ -348,8 +503,10 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
         }
         return;
       }
+      phase(st_tpop);
       // ---- generate the children, 64 per round ---------------------------
       for (uint32_t base = 0; base < total; base += kWave) {
+        ++st_rounds;
         const uint32_t c = base + lane;
         if (c < total) {
           int lo = 0, hi = n - 1;  // parent p: s_off[p] <= c < s_off[p+1]
@@ -365,12 +522,12 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
             const uint32_t par[5] = {pr[0], pr[1], pr[2], pr[3], pr[4]};
             uts_spawn(par, i, ch);
 #pragma unroll
-            for (int w = 0; w < 5; ++w) deq[w * dq_cap + at] = ch[w];
-            deq[5 * dq_cap + at] = pr[5] + 1;  // depth + 1, same slot
+            for (int w = 0; w < 5; ++w) deq[dq_idx<W>(at, w)] = ch[w];
+            deq[dq_idx<W>(at, 5)] = pr[5] + 1;  // depth + 1, same slot
           } else {
             const uint32_t node = pr[0];
-            deq[at] = (c - s_off[lo]) == 0 ? (uint32_t)a.child_a[node] : (uint32_t)a.child_b[node];
-            deq[dq_cap + at] = pr[1];
+            deq[dq_idx<W>(at, 0)] = (c - s_off[lo]) == 0 ? (uint32_t)a.child_a[node] : (uint32_t)a.child_b[node];
+            deq[dq_idx<W>(at, 1)] = pr[1];
           }
         }
       }
@@ -378,63 +535,17 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
       ++batches;
       st_nodes += n;
       __syncthreads();
-      // ---- donate the oldest half while others are hungry ----------------
-      if ((batches & 3) == 0 && top - bot >= 2 * kWave) {
-        const uint32_t hg = lane == 0 ? ld_rlx(a.hungry) : 0;
-        if (__shfl(hg, 0, 64) != 0) {
-          const long k0 = (top - bot) / 2;
-          const int p = (home + batches / 4) % a.np;
-          uint32_t *meta_p = a.pool_meta + (long)p * 64;
-          __shared__ int s_k, s_base;
-          if (lane == 0) {
-            int k = -1, basei = 0;  // -1: lock not taken
-            if (pool_lock(meta_p, g)) {
-              const uint32_t cnt = ld_rlx(meta_p + 32);
-              k = (int)std::min<long>(k0, (long)a.pc - cnt);
-              if (k < 0) k = 0;
-              basei = (int)cnt;
-              st_rlx(meta_p + 32, cnt + (uint32_t)k);
-            }
-            s_k = k;
-            s_base = basei;
-          }
-          __syncthreads();
-          const int k = s_k;
-          if (k > 0) {
-            uint32_t *pool = a.pool_rec + (long)p * a.pc * W;
-            for (int i = lane; i < k; i += kWave) {
-              const long at = (bot + i) & dq_mask;
-              const uint32_t sl = meta_slot(deq[(W - 1) * dq_cap + at]);
-#pragma unroll
-              for (int w = 0; w < W; ++w) pool[(long)w * a.pc + s_base + i] = deq[w * dq_cap + at];
-              atomicAdd(&s_hold[sl], -1);
-              atomicAdd(&s_cred[sl], -1);  // transferred to the pool (may go < 0)
-            }
-            bot += k;
-            ++st_don;
-            st_donr += k;
-            __syncthreads();
-            // keep >= 1 credit for every slot still held or with counts not
-            // yet flushed (settle flushes them under that credit); buy the rest
-            for (int s = lane; s < kMaxSlots; s += kWave) {
-              const int want = (s_hold[s] > 0 || s_nodes[s] != 0) ? 1 : 0;
-              if (s_cred[s] < want) {
-                __hip_atomic_fetch_add(a.credit + (long)s * HS_LINE_WORDS,
-                                       (uint32_t)(want - s_cred[s]), __ATOMIC_RELAXED, HS_AGENT);
-                s_cred[s] = want;
-              }
-            }
-            __syncthreads();
-            if (lane == 0) {
-              fence_rel();  // pool records and credits before the unlock
-              st_rel(meta_p, 0);
-            }
-          } else if (lane == 0 && k == 0) {
-            st_rel(meta_p, 0);  // pool full: nothing moved
-          }
-          __syncthreads();
-        }
+      phase(st_tgen);
+      // ---- hand-offs: a late delivery in, the oldest half out ---------------
+      {
+        const int k_in = (int)__shfl(box_n, 0, 64);
+        if (k_in > 0 && !absorb(k_in)) return;
       }
+      if ((batches & 1) == 0 && top - bot >= 2 * kWave) donate();
+      // return the credits of trees this WG no longer holds, so that a tree
+      // finishes while its last holders work on other trees
+      if (SERVE && (batches & 7) == 0) settle();
+      phase(st_tdon);
       if ((batches & 63) == 0 && g.aborted()) {
         flush_stats();
         return;
@@ -445,53 +556,12 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
 
     // ---- no local work: settle, then look for more -------------------------
     settle();
-    __shared__ int s_got;
-    if (lane == 0) {
-      s_got = 0;
-      for (int t = 0; t < a.np && !s_got; ++t) {
-        const int p = (home + t) % a.np;
-        uint32_t *meta_p = a.pool_meta + (long)p * 64;
-        if (ld_rlx(meta_p + 32) == 0) continue;
-        if (!pool_lock(meta_p, g)) break;
-        const uint32_t cnt = ld_rlx(meta_p + 32);
-        const int k = (int)std::min<uint32_t>(cnt, (uint32_t)(dq_cap / 4));
-        if (k > 0) {
-          st_rlx(meta_p + 32, cnt - (uint32_t)k);
-          s_got = k;
-          s_pst[0] = (uint32_t)p;
-          s_pst[1] = cnt - (uint32_t)k;
-        } else {
-          st_rel(meta_p, 0);
-        }
+    {
+      const int k_in = (int)__shfl(lane == 0 ? ld_rlx(my_n) : 0u, 0, 64);
+      if (k_in > 0) {
+        if (!absorb(k_in)) return;
+        continue;
       }
-    }
-    __syncthreads();
-    const int got = s_got;
-    if (got > 0) {
-      const int p = (int)s_pst[0];
-      const long from = s_pst[1];
-      const uint32_t *pool = a.pool_rec + (long)p * a.pc * W;
-      __syncthreads();
-      for (int i = lane; i < got; i += kWave) {
-        const long at = (top + i) & dq_mask;
-        uint32_t last = 0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-          last = pool[(long)w * a.pc + from + i];
-          deq[w * dq_cap + at] = last;
-        }
-        const uint32_t sl = meta_slot(last);
-        atomicAdd(&s_hold[sl], 1);
-        atomicAdd(&s_cred[sl], 1);  // pool records carry their credits
-      }
-      top += got;
-      ++st_steal;
-      st_stolen += got;
-      __syncthreads();
-      if (lane == 0) st_rel(a.pool_meta + (long)p * 64, 0);
-      if (hungry && lane == 0) add_rlx(a.hungry, 0xffffffffu);
-      hungry = false;
-      continue;
     }
     if (SERVE) {  // admit the next submitted request, if any
       __shared__ int s_adm;
@@ -511,6 +581,7 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
           __hip_atomic_store(a.leafsum + slot, 0ull, __ATOMIC_RELAXED, HS_AGENT);
           st_rlx(a.maxd + slot, 0);
           st_rlx(a.slot_gen_mx + slot, gmx);
+          s_gmx[slot] = gmx;
           st_rlx(a.credit + (long)slot * HS_LINE_WORDS, 1);
           add_rlx(a.live, 1);
           s_pst[0] = slot;
@@ -526,35 +597,47 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
           uts_root(s_pst[1], st);
           const long at = top & dq_mask;
 #pragma unroll
-          for (int w = 0; w < 5; ++w) deq[w * dq_cap + at] = st[w];
-          deq[(W - 1) * dq_cap + at] = slot << 16;
+          for (int w = 0; w < 5; ++w) deq[dq_idx<W>(at, w)] = st[w];
+          deq[dq_idx<W>(at, W - 1)] = slot << 16;
           s_hold[slot] = 1;
           s_cred[slot] = 1;
         }
         top += 1;
         __syncthreads();
-        if (hungry && lane == 0) add_rlx(a.hungry, 0xffffffffu);
-        hungry = false;
         continue;
       }
     }
-    if (!hungry && lane == 0) add_rlx(a.hungry, 1);
-    hungry = true;
+    // post one work request per idle period: take a ring slot (release:
+    // the cleared inbox count and the inbox reads come first), then write
+    // this WG's id into it
+    if (!requested) {
+      if (lane == 0) {
+        const int r = me & (a.nrq - 1);
+        const uint32_t t = __hip_atomic_fetch_add(a.rq + (long)r * HS_LINE_WORDS + 1, 1u,
+                                                  __ATOMIC_RELEASE, HS_AGENT);
+        st_rlx(a.rq_ring + (long)r * (a.rq_mask + 1) + (t & a.rq_mask), (uint32_t)me + 1);
+      }
+      requested = true;
+    }
     // finished?  single tree: every credit returned; serving: host stop
     __shared__ int s_exit;
     if (lane == 0) {
-      bool fin;
-      if (SERVE)
-        fin = __hip_atomic_load(a.stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != 0 &&
-              ld_rlx(a.live) == 0;
-      else
-        fin = ld_rlx(a.credit) == 0;
+      // the termination words are shared by every idle WG: look every 8th pass
+      bool fin = false;
+      if ((idle_iter++ & 7) == 0) {
+        if (SERVE)
+          fin = __hip_atomic_load(a.stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != 0 &&
+                ld_rlx(a.live) == 0;
+        else
+          fin = ld_rlx(a.credit) == 0;
+      }
       if (!fin && !g.ok()) fin = true;
-      if (!fin) sleep_quanta(8);
+      if (!fin) sleep_quanta(idle_q);
       s_exit = fin;
     }
     __syncthreads();
     tick(false);
+    idle_q = idle_q >= 32 ? 32 : 2 * idle_q;
     if (s_exit) {
       flush_stats();
       return;
@@ -605,17 +688,28 @@ struct Engine {
   uint32_t *d_thr = nullptr;
 
   // tree: 0 reference, 1 geometric
-  int setup(hs_ctx *c, int tree, int num_blocks, int dq_log2, int np, int pc, double b0) {
+  int setup(hs_ctx *c, int tree, int num_blocks, int dq_log2, double b0) {
     std::memset(&a, 0, sizeof a);
     blocks = num_blocks > 0 ? num_blocks : 4 * c->num_cu;
     W = tree == 1 ? 6 : 2;
     a.dq_log2 = dq_log2;
-    a.np = np;
-    a.pc = pc;
+    // one inbox holds at most a quarter deque, so a late delivery fits
+    a.wbox_cap = (int)std::min<long>(4096, (1L << dq_log2) / 4);
+    // request rings: up to 32; every WG has at most one request outstanding
+    // on its ring, and a claimed entry may be cleared late: twice the ring's
+    // WG count keeps a new request off it
+    a.nrq = 1;
+    while (a.nrq < 32 && a.nrq * 8 < blocks) a.nrq <<= 1;
+    const uint32_t per_ring = (uint32_t)((blocks + a.nrq - 1) / a.nrq);
+    uint32_t ring = 1;
+    while (ring < 2u * per_ring) ring <<= 1;
+    a.rq_mask = ring - 1;
     int rc;
-    if ((rc = ar.alloc(&a.deq, ((size_t)blocks << dq_log2) * W))) return rc;
-    if ((rc = ar.alloc(&a.pool_rec, (size_t)np * pc * W))) return rc;
-    if ((rc = ar.alloc(&a.pool_meta, (size_t)np * 64))) return rc;
+    if ((rc = ar.alloc(&a.deq, (size_t)blocks * (((size_t)W << dq_log2) + kDqSkew)))) return rc;
+    if ((rc = ar.alloc(&a.wbox, (size_t)blocks * a.wbox_cap * W))) return rc;
+    if ((rc = ar.alloc(&a.wbox_n, (size_t)blocks * HS_LINE_WORDS))) return rc;
+    if ((rc = ar.alloc(&a.rq, (size_t)a.nrq * HS_LINE_WORDS))) return rc;
+    if ((rc = ar.alloc(&a.rq_ring, (size_t)a.nrq * ring))) return rc;
     if ((rc = ar.alloc(&a.credit, (size_t)kMaxSlots * HS_LINE_WORDS))) return rc;
     if ((rc = ar.alloc(&a.nodes, kMaxSlots))) return rc;
     if ((rc = ar.alloc(&a.leaves, kMaxSlots))) return rc;
@@ -623,18 +717,18 @@ struct Engine {
     if ((rc = ar.alloc(&a.maxd, kMaxSlots))) return rc;
     if ((rc = ar.alloc(&a.slot_gen_mx, kMaxSlots))) return rc;
     if ((rc = ar.alloc(&d_thr, kMaxThr))) return rc;
-    if ((rc = ar.alloc(&a.hungry, 64))) return rc;
     if ((rc = ar.alloc(&a.ctl, 64))) return rc;
     if ((rc = ar.alloc(&a.live, 64))) return rc;
     a.admitted = a.live + 32;
-    HS_HIP(hipMemset(a.pool_meta, 0, (size_t)np * 64 * 4));
+    HS_HIP(hipMemset(a.wbox_n, 0, (size_t)blocks * HS_LINE_WORDS * 4));
+    HS_HIP(hipMemset(a.rq, 0, (size_t)a.nrq * HS_LINE_WORDS * 4));
+    HS_HIP(hipMemset(a.rq_ring, 0, (size_t)a.nrq * ring * 4));
     HS_HIP(hipMemset(a.credit, 0, (size_t)kMaxSlots * HS_LINE_WORDS * 4));
     HS_HIP(hipMemset(a.nodes, 0, kMaxSlots * 8));
     HS_HIP(hipMemset(a.leaves, 0, kMaxSlots * 8));
     HS_HIP(hipMemset(a.leafsum, 0, kMaxSlots * 8));
     HS_HIP(hipMemset(a.maxd, 0, kMaxSlots * 4));
     HS_HIP(hipMemset(a.slot_gen_mx, 0, kMaxSlots * 4));
-    HS_HIP(hipMemset(a.hungry, 0, 256));
     HS_HIP(hipMemset(a.ctl, 0, 256));
     HS_HIP(hipMemset(a.live, 0, 256));
     if (tree == 1) {
@@ -648,12 +742,12 @@ struct Engine {
     return HS_OK;
   }
 
-  // Puts one record (W words) into pool 0 with its credit, as a root.
+  // Delivers one root record (W words) with its credit to WG 0's inbox.
   int seed_root(const uint32_t *rec, int slot) {
-    for (int w = 0; w < W; ++w)  // entry 0 of pool 0, word-major
-      HS_HIP(hipMemcpy(a.pool_rec + (size_t)w * a.pc, rec + w, 4, hipMemcpyHostToDevice));
+    for (int w = 0; w < W; ++w)  // record 0 of the inbox: word w at w * 64
+      HS_HIP(hipMemcpy(a.wbox + (size_t)w * 64, rec + w, 4, hipMemcpyHostToDevice));
     const uint32_t one = 1;
-    HS_HIP(hipMemcpy(a.pool_meta + 32, &one, 4, hipMemcpyHostToDevice));
+    HS_HIP(hipMemcpy(a.wbox_n, &one, 4, hipMemcpyHostToDevice));
     HS_HIP(hipMemcpy(a.credit + (size_t)slot * HS_LINE_WORDS, &one, 4, hipMemcpyHostToDevice));
     return HS_OK;
   }
@@ -702,12 +796,15 @@ int stats_report(Engine &e, double kernel_ms) {
   std::fprintf(stderr,
                "uts stats: %d WGs, %.3f ms: busy %.1f%% (%.3f ms/WG), batches %llu, "
                "nodes/batch %.1f, donations %llu (%.1f rec), steals %llu (%.1f rec), "
-               "nodes/WG min %llu med %llu max %llu\n",
+               "nodes/WG min %llu med %llu max %llu; per batch: pop %.2f us, generate %.2f us "
+               "(%.2f rounds), donate %.2f us\n",
                e.blocks, kernel_ms, 100.0 * busy / std::max(1.0, busy + idle),
                busy / 1e5 / e.blocks, sum[2], (double)sum[3] / std::max(1ull, sum[2]), sum[4],
                (double)sum[5] / std::max(1ull, sum[4]), sum[6],
                (double)sum[7] / std::max(1ull, sum[6]), nodes.front(), nodes[nodes.size() / 2],
-               nodes.back());
+               nodes.back(), sum[8] / 100.0 / std::max(1ull, sum[2]),
+               sum[9] / 100.0 / std::max(1ull, sum[2]), (double)sum[11] / std::max(1ull, sum[2]),
+               sum[10] / 100.0 / std::max(1ull, sum[2]));
   return HS_OK;
 }
 
@@ -747,7 +844,7 @@ extern "C" int hs_uts_ref_run(hs_ctx *c, int num_nodes, const uint32_t *value,
       return HS_EINVAL;
   HS_HIP(hipSetDevice(c->device));
   Engine e;
-  int rc = e.setup(c, 0, num_blocks, 14, 64, std::max(1024, num_nodes), 0);
+  int rc = e.setup(c, 0, num_blocks, 14, 0);
   if (rc) return rc;
   uint32_t *d_val;
   int32_t *d_a, *d_b;
@@ -772,7 +869,7 @@ extern "C" int hs_uts_geo_run(hs_ctx *c, int seed, double b0, int gen_mx, int nu
   std::memset(res, 0, sizeof *res);
   HS_HIP(hipSetDevice(c->device));
   Engine e;
-  int rc = e.setup(c, 1, num_blocks, 16, 64, 1 << 16, b0);
+  int rc = e.setup(c, 1, num_blocks, 16, b0);
   if (rc) return rc;
   const uint32_t g = (uint32_t)gen_mx;
   HS_HIP(hipMemcpy(e.a.slot_gen_mx, &g, 4, hipMemcpyHostToDevice));
@@ -803,7 +900,7 @@ extern "C" int hs_uts_serve(hs_ctx *c, const hs_uts_request *req, int n, const h
   const int slots = cfg->slots > 0 ? std::min(cfg->slots, kMaxSlots) : 8;
   HS_HIP(hipSetDevice(c->device));
   Engine e;
-  int rc = e.setup(c, 1, cfg->num_blocks, 16, 64, 1 << 16, b0);
+  int rc = e.setup(c, 1, cfg->num_blocks, 16, b0);
   if (rc) return rc;
   if (cfg->timeout_s > 0) e.a.ticks = timeout_ticks(c, cfg->timeout_s);
   // pinned, host-coherent request inbox / response board / stop word

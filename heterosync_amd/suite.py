@@ -37,13 +37,19 @@ def c1(dev, oh):
             "acquires": r.cs_entries, "parity_ok": ok and r.state_ok == 1}
 
 
+SRB_NAMES = ("atomicTreeBarrUniqSRB", "atomicTreeBarrUniqLocalExchSRB",
+             "atomicTreeBarrUniqNaiveSRB", "atomicTreeBarrUniqLocalExchNaiveSRB",
+             "atomicTreeBarrUniqNaiveAllSRB", "atomicTreeBarrUniqLocalExchNaiveAllSRB")
+
+
 def c2(dev, oh, scopes=(0, 1)):
     out = {}
     ncu = dev.info.num_cu
     for name in ("atomicTreeBarrUniq", "atomicTreeBarrUniqLocalExch", "lfTreeBarrUniq",
-                 "lfTreeBarrUniqLocalExch"):
+                 "lfTreeBarrUniqLocalExch") + SRB_NAMES:
         pid = hs.prim_id(name)
-        for sc in scopes:
+        # the SRB family is flat (release/acquire) at 4 WGs per CU
+        for sc in (scopes if pid < 4 else (1,)):
             s, r = dev.syncprim(pid, 10, 4 * ncu, 1000, wg_size=256, reps=10, timeout_s=30.0,
                                 sync_scope=sc)
             ok = bool(np.array_equal(s, oh.golden(pid, 10, 4 * ncu, 1000, 10, 256, ncu, 0)))

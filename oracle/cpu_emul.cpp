@@ -85,10 +85,14 @@ extern "C" double hso_cpu_run(int prim, int L, int num_wgs, int iters,
   for (long i = 0; i < n; ++i) s[i] = (int32_t)i;
   const int nab = num_wgs < num_cu ? num_wgs : num_cu;
   const int perCU = ceil_div(num_wgs, nab);
-  const bool barrier = prim < 4;
-  const bool lx = prim == 1 || prim == 3;
+  const bool srb = prim >= 37 && prim <= 42;  // sense-reversing family
+  const bool barrier = prim < 4 || srb;
+  const bool lx = prim == 1 || prim == 3 || prim == 38 || prim == 40 || prim == 42;
   const bool mutex = prim >= 4 && prim <= 7;
-  const bool sem = prim >= 8 && prim <= 15;
+  /* global semaphores, including both priority families (16-19 spin,
+   * 32-35 EBO; HM:2408-2470 dispatch both to the global kernels) */
+  const bool prio = (prim >= 16 && prim <= 19) || (prim >= 32 && prim <= 35);
+  const bool sem = (prim >= 8 && prim <= 19) || (prim >= 32 && prim <= 35);
   const bool umutex = prim >= 20 && prim <= 23;
   const bool usem = prim >= 24 && prim <= 31;
   if (!(barrier || mutex || sem || umutex || usem)) return -1.0;
@@ -97,8 +101,8 @@ extern "C" double hso_cpu_run(int prim, int L, int num_wgs, int iters,
 
   static const int kSemMax[4] = {1, 2, 10, 120};
   const int kind = mutex ? prim - 4 : umutex ? prim - 20 : 0;
-  const int sem_n = sem ? kSemMax[(prim - 8) % 4] : usem ? kSemMax[(prim - 24) % 4] : 0;
-  const bool sem_ebo = (sem && prim >= 12) || (usem && prim >= 28);
+  const int sem_n = (sem || usem) ? kSemMax[(prim - 8) % 4] : 0;
+  const bool sem_ebo = (prim >= 12 && prim <= 15) || (usem && prim >= 28) || prim >= 32;
 
   // lock state: index 0 = global, 1 + c = per group c
   const int nlocks = 1 + num_cu;
@@ -113,7 +117,7 @@ extern "C" double hso_cpu_run(int prim, int L, int num_wgs, int iters,
     }
   }
   struct SemRec {
-    Word lock, count, waiting;
+    Word lock, count, waiting, prio;
     uint32_t max;
   };
   std::vector<SemRec> sems(nlocks);
@@ -127,6 +131,12 @@ extern "C" double hso_cpu_run(int prim, int L, int num_wgs, int iters,
   // barrier state
   GenBarrier global;
   global.n = nab;
+  // SRB: flat barriers over every WG (the NaiveAll forms count threads, which
+  // orders exactly the same WGs); 37/38 are a tree only above 4 WGs per group
+  // (hipLocksBarrierAtomicSRB.h:186-225)
+  GenBarrier flat, flat_local;
+  flat.n = flat_local.n = num_wgs;
+  const bool srb_tree = (prim == 37 || prim == 38) && perCU > 4;
   std::vector<GenBarrier> local(nab);
   for (int c = 0; c < nab; ++c) local[c].n = perCU;
   // LF barrier: epoch flags
@@ -183,13 +193,20 @@ extern "C" double hso_cpu_run(int prim, int L, int num_wgs, int iters,
         for (int i = 0; i < iters; ++i) {
           wg_access(s, (long)cur * wg, wg, L, mode);
           if (lx && perCU > 1) {
-            // local exchange: local barrier then the neighbour's region
-            local[b % nab].arrive();
-            cur = (cur + nab) % num_wgs;
+            // local exchange: local barrier then the neighbour's region;
+            // NaiveSRB's exchange stride is the grid, i.e. its own region
+            // (main.hip.cpp:472-474, 1568)
+            if (prim == 40 || prim == 42)
+              flat_local.arrive();
+            else
+              local[b % nab].arrive();
+            if (prim != 40) cur = (cur + nab) % num_wgs;
             wg_access(s, (long)cur * wg, wg, L, mode);
           }
-          if (prim <= 1)
+          if (prim <= 1 || (srb && srb_tree))
             atomic_barrier(b);
+          else if (srb)
+            flat.arrive();
           else
             lf_barrier(b, (uint32_t)(rep * iters + i + 1));
           cur = (cur + 1) % num_wgs;
@@ -241,6 +258,8 @@ extern "C" double hso_cpu_run(int prim, int L, int num_wgs, int iters,
     for (int k = 0; k < iters * reps; ++k) {
       int bo = 1;
       for (;;) {  // TryWait, HSS:32-141
+        if (prio)  // yield to posts in flight, HSS:464-469
+          while (r.prio.v.load(std::memory_order_relaxed) != 0) std::this_thread::yield();
         uint32_t z = 0;
         if (!r.lock.v.compare_exchange_strong(z, 1, std::memory_order_acquire)) {
           std::this_thread::yield();
@@ -286,6 +305,7 @@ extern "C" double hso_cpu_run(int prim, int L, int num_wgs, int iters,
         volatile int32_t sink = acc;
         (void)sink;
       }
+      if (prio) r.prio.v.fetch_add(1, std::memory_order_relaxed);  // HSS:595
       for (;;) {  // Post, HSS:160-232
         uint32_t z = 0;
         if (r.lock.v.compare_exchange_strong(z, 1, std::memory_order_acquire)) break;
@@ -294,6 +314,7 @@ extern "C" double hso_cpu_run(int prim, int L, int num_wgs, int iters,
       uint32_t cnt = r.count.v.load(std::memory_order_relaxed);
       r.count.v.store(writer ? cnt + r.max : cnt + 1, std::memory_order_relaxed);
       r.lock.v.store(0, std::memory_order_release);
+      if (prio) r.prio.v.fetch_sub(1, std::memory_order_relaxed);  // HSS:640
     }
   };
 

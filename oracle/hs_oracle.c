@@ -187,16 +187,18 @@ int hso_region_passes(int p, int num_wgs, int iters, int reps, int num_cu,
     for (int r = 0; r < m; ++r) passes[r] = ir * ((is_lx(p) && perCU > 1) ? 2 : 1);
   } else if (is_mutex(p)) {
     passes[0] = (int64_t)num_wgs * ir;
-  } else if (is_sem(p)) {
-    if (p >= 16) return -1; /* priority variants: not built */
+  } else if (is_sem(p) || (p >= 32 && p <= 35)) {
+    /* global semaphores; the priority EBO ids 32-35 run the global kernel
+     * too (HM:2455-2470 -> invokeEBOSemaphorePriority) over storage sized
+     * like the uniq semaphores' (HM:2109-2128), so only the first perCU of
+     * their num_wgs regions are written */
     int64_t writers = 0;
     for (int b = 0; b < num_wgs; ++b)
       if (b / nab == (b % nab) % perCU) ++writers;
-    for (int r = 0; r < m; ++r) passes[r] = writers * ir;
+    for (int r = 0; r < m && r < perCU; ++r) passes[r] = writers * ir;
   } else if (is_uniq_mutex(p)) {
     for (int b = 0; b < num_wgs; ++b) passes[b % num_cu] += ir;
   } else if (is_uniq_sem(p)) {
-    if (p >= 32) return -1;
     if (num_wgs % nab) return -1; /* every group needs its writer */
     for (int b = 0; b < num_wgs; ++b) {
       if (b / nab != (b % nab) % perCU) continue;

@@ -35,7 +35,7 @@ def test_prim_table_and_sizes(oracle):
         assert hs.prim_name(pid) == name
     assert hs.prim_id("bogus") == -1
     lib = hs.lib()
-    supported = list(range(16)) + list(range(20, 32)) + list(range(37, 43))
+    supported = list(range(36)) + list(range(37, 43))
     assert [p for p in range(47) if lib.hs_prim_supported(p)] == supported
     for pid in supported:
         for nw, ncu, wg, L in [(32, 256, 64, 10), (1024, 256, 256, 10), (3, 4, 64, 0),

@@ -17,8 +17,11 @@ import pytest
 P8 = ctypes.POINTER(ctypes.c_uint8)
 P32 = ctypes.POINTER(ctypes.c_int32)
 
-SUPPORTED = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15,
-             20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31]
+SUPPORTED = list(range(36)) + list(range(37, 43))
+# priority EBO semaphores: the reference dispatches them to the global kernel
+# (main.hip.cpp:2455-2470) but checks them with the uniq golden
+# (main.hip.cpp:2667-2680), so its own check disagrees with its kernel
+PRIO_EBO = [32, 33, 34, 35]
 
 REF_NAMES = {  # hip/syncPrims/uvm/main.hip.cpp:2036-2089
     "atomicTreeBarrUniq": 0, "atomicTreeBarrUniqLocalExch": 1, "lfTreeBarrUniq": 2,
@@ -208,12 +211,13 @@ def test_prim_table_matches_reference(oracle):
 
 def _simulate(prim, L, nw, iters, reps, wg, ncu, mode):
     """Brute force: replay the kernels' per-WG region walk, one pass at a time."""
-    n = None
-    if prim < 4 or 24 <= prim <= 31:
+    barrier = prim < 4 or 37 <= prim <= 42
+    gsem = 8 <= prim <= 19 or 32 <= prim <= 35   # global semaphore kernels
+    if barrier or 24 <= prim <= 35:
         mult = nw
     elif prim <= 7:
         mult = 1
-    elif prim <= 15:
+    elif prim <= 19:
         mult = -(-nw // ncu)
     else:
         mult = min(nw, ncu)
@@ -221,6 +225,7 @@ def _simulate(prim, L, nw, iters, reps, wg, ncu, mode):
     s = np.arange(n, dtype=np.int64)
     nab = min(nw, ncu)
     per = -(-nw // nab)
+    lx = prim in (1, 3, 38, 40, 42) and per > 1
 
     def one_pass(region):
         for t in range(wg):
@@ -236,16 +241,18 @@ def _simulate(prim, L, nw, iters, reps, wg, ncu, mode):
 
     for _ in range(reps):
         for i in range(iters):
-            if prim < 4:
+            if barrier:
+                # exchange stride: the group count, or the grid for NaiveSRB
+                xs = 0 if prim == 40 else nab
                 for b in range(nw):  # episode i: WG b at region (b + shift)
-                    cur = (b + i * ((nab + 1) if (prim in (1, 3) and per > 1) else 1)) % nw
+                    cur = (b + i * ((xs + 1) if lx else 1)) % nw
                     one_pass(cur)
-                    if prim in (1, 3) and per > 1:
-                        one_pass((cur + nab) % nw)
+                    if lx:
+                        one_pass((cur + xs) % nw)
             elif prim <= 7:
                 for b in range(nw):
                     one_pass(0)
-            elif prim <= 15:
+            elif gsem:
                 for b in range(nw):
                     if b // nab == (b % nab) % per:
                         for j in range(per):
@@ -266,7 +273,7 @@ def _simulate(prim, L, nw, iters, reps, wg, ncu, mode):
 SHAPES = [(4, 4), (8, 4), (2, 4), (12, 4), (16, 8)]
 
 
-@pytest.mark.parametrize("prim", SUPPORTED)
+@pytest.mark.parametrize("prim", [p for p in SUPPORTED if p not in PRIO_EBO])
 def test_literal_golden_equals_closed_form(ohelp, prim):
     for nw, ncu in SHAPES:
         for L in (0, 1, 3):
@@ -289,6 +296,22 @@ def test_closed_form_equals_bruteforce_walk(ohelp, prim):
                 continue
             sim = _simulate(prim, 2, nw, 2, 2, 64, ncu, mode)
             assert np.array_equal(sim, closed), (prim, nw, ncu, mode)
+
+
+@pytest.mark.parametrize("prim", PRIO_EBO)
+def test_prio_ebo_reference_golden_disagrees_with_its_kernel(ohelp, prim):
+    """The reference's golden for 32-35 is the uniq one (every region once per
+    iteration); its kernel is the global one (each group's writer walks
+    regions 0..perCU-1).  The oracle follows the kernel: regions from perCU
+    on keep their initial values."""
+    for nw, ncu in [(4, 4), (8, 4), (16, 8)]:
+        closed = ohelp.golden(prim, 2, nw, 3, 2, 64, ncu, 1)
+        lit = ohelp.golden(prim, 2, nw, 3, 2, 64, ncu, 1, literal=True)
+        assert not np.array_equal(lit, closed)
+        per = -(-nw // min(nw, ncu))
+        region = 64 * 3
+        assert np.array_equal(closed[per * region:], np.arange(per * region, closed.size))
+        assert np.array_equal(closed, _simulate(prim, 2, nw, 3, 2, 64, ncu, 1))
 
 
 def test_planted_mistakes_are_caught(ohelp):

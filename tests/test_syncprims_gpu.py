@@ -12,6 +12,7 @@ pytestmark = pytest.mark.gpu
 MUTEX = [4, 5, 6, 7]
 UMUTEX = [20, 21, 22, 23]
 SEM = list(range(8, 16))
+PRIO = [16, 17, 18, 19, 32, 33, 34, 35]
 USEM = list(range(24, 32))
 BARRIER = [0, 1, 2, 3]
 SRB = [37, 38, 39, 40, 41, 42]
@@ -55,6 +56,18 @@ def test_semaphore(dev, ohelp, prim):
     ncu = dev.info.num_cu
     check(dev, ohelp, prim, 4, 16, 10, mode=1)
     check(dev, ohelp, prim, 4, 2 * ncu, 2, reps=2, mode=1)  # 1 writer + readers per CU
+
+
+@pytest.mark.parametrize("prim", PRIO)
+@pytest.mark.parametrize("mode", [0, 1])
+def test_priority_semaphore(dev, ohelp, prim, mode):
+    """Entries yield to posts in flight; 32-35 run the global kernel over
+    num_wgs regions (the reference's dispatch), so only regions < perCU move."""
+    ncu = dev.info.num_cu
+    check(dev, ohelp, prim, 4, 16, 10, mode=mode)
+    check(dev, ohelp, prim, 4, 2 * ncu, 2, reps=2, mode=mode)
+    r = check(dev, ohelp, prim, 2, 4 * ncu, 3, reps=2, mode=mode)
+    assert r.state_ok == 1
 
 
 @pytest.mark.parametrize("prim", USEM)
@@ -115,8 +128,8 @@ def test_fa_mutex_c1(dev, ohelp):
 def test_error_paths(dev):
     ncu = dev.info.num_cu
     with pytest.raises(hs.HSError) as e:
-        dev.syncprim(16, 10, 8, 1)           # priority semaphore: not built
-    assert e.value.code == -7
+        dev.syncprim(36, 10, 8, 1)           # id 36 is reserved in the reference
+    assert e.value.code == -1
     with pytest.raises(hs.HSError) as e:
         dev.syncprim(0, 10, ncu + 1, 1)      # barrier grid not a multiple of NUM_CU
     assert e.value.code == -1
