@@ -18,6 +18,9 @@ struct hs_ctx {
   double wall_hz = 0;  // wall_clock64() ticks per second
   char arch[64] = {0};
   hipStream_t stream = nullptr;
+  // engine kept between UTS calls, released by hs_close
+  void *uts_cache = nullptr;
+  void (*uts_cache_free)(void *) = nullptr;
 };
 
 namespace hs {

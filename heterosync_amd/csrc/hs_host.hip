@@ -118,6 +118,7 @@ void hs_close(hs_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->uts_cache_free) c->uts_cache_free(c->uts_cache);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -56,9 +56,22 @@ constexpr int kMaxThr = 128;
 constexpr long kGrow = 1024;  // deque size below which a WG pops oldest first
 
 // ---------------------------------------------------------------------------
-// SHA-1 (FIPS 180-4) of one padded 64-byte block.
+// SHA-1 (FIPS 180-4) of one padded 64-byte block.  The round functions and
+// the message-schedule XORs use gfx950's three-input logic op, v_bitop3_b32,
+// whose 8-bit immediate is the truth table over inputs (x, y, z) weighted
+// 0xF0, 0xCC, 0xAA: Ch = 0xCA, parity = 0x96, Maj = 0xE8.
 __device__ __forceinline__ uint32_t rotl(uint32_t x, int n) {
   return __builtin_amdgcn_alignbit(x, x, 32 - n);
+}
+
+__device__ __forceinline__ uint32_t lop3_ch(uint32_t x, uint32_t y, uint32_t z) {
+  return __builtin_amdgcn_bitop3_b32(x, y, z, 0xCA);  // (x & y) | (~x & z)
+}
+__device__ __forceinline__ uint32_t lop3_xor(uint32_t x, uint32_t y, uint32_t z) {
+  return __builtin_amdgcn_bitop3_b32(x, y, z, 0x96);  // x ^ y ^ z
+}
+__device__ __forceinline__ uint32_t lop3_maj(uint32_t x, uint32_t y, uint32_t z) {
+  return __builtin_amdgcn_bitop3_b32(x, y, z, 0xE8);  // majority
 }
 
 __device__ __forceinline__ void sha1_block(uint32_t h[5], uint32_t w[16]) {
@@ -70,21 +83,21 @@ __device__ __forceinline__ void sha1_block(uint32_t h[5], uint32_t w[16]) {
     if (t < 16) {
       wt = w[t];
     } else {
-      wt = rotl(w[(t - 3) & 15] ^ w[(t - 8) & 15] ^ w[(t - 14) & 15] ^ w[t & 15], 1);
+      wt = rotl(lop3_xor(w[(t - 3) & 15], w[(t - 8) & 15], w[(t - 14) & 15]) ^ w[t & 15], 1);
       w[t & 15] = wt;
     }
     uint32_t f, k;
     if (t < 20) {
-      f = (b & c) | (~b & d);
+      f = lop3_ch(b, c, d);
       k = 0x5a827999u;
     } else if (t < 40) {
-      f = b ^ c ^ d;
+      f = lop3_xor(b, c, d);
       k = 0x6ed9eba1u;
     } else if (t < 60) {
-      f = (b & c) | (b & d) | (c & d);
+      f = lop3_maj(b, c, d);
       k = 0x8f1bbcdcu;
     } else {
-      f = b ^ c ^ d;
+      f = lop3_xor(b, c, d);
       k = 0xca62c1d6u;
     }
     const uint32_t tmp = rotl(a, 5) + f + e + k + wt;
@@ -161,11 +174,15 @@ struct UtsArgs {
   // serving (null when not serving)
   uint32_t *live;          // number of live (admitted, unfinished) trees
   uint32_t *admitted;      // requests taken from the inbox (device)
-  // pinned host memory, system scope:
+  // pinned host memory, system scope; only the dispatcher WG reads it:
   const uint32_t *inbox;   // per request: slot, seed, gen_mx, pad
   const uint32_t *submitted;
   uint32_t *resp;          // per slot: 8 words (nodes lo/hi, leaves lo/hi, maxd, done seq)
   const uint32_t *stop;
+  // device copies kept by the dispatcher: requests (4 words each), the
+  // submitted count (dev_ctl[0]) and the stop word (dev_ctl[32])
+  uint32_t *dev_inbox;
+  uint32_t *dev_ctl;
   // optional per-WG statistics (null: off), kStatWords each:
   // busy ticks, idle ticks, batches, nodes, donations, donated, steals,
   // stolen, pop ticks, generate ticks, donate ticks, generate rounds
@@ -218,6 +235,37 @@ __device__ __forceinline__ long dq_idx(long at, int w) {
   return ((at >> 6) * W + w) * 64 + (at & 63);
 }
 
+// Serving: the one WG that reads host memory.  It copies newly submitted
+// requests into device memory and publishes their count, and forwards the
+// host's stop word; the workers poll only these device copies, so host
+// memory sees one poller however many WGs are idle.  Whole wave.
+__device__ void dispatch(const UtsArgs &a, Guard &g) {
+  const int lane = threadIdx.x;
+  const int sys = __HIP_MEMORY_SCOPE_SYSTEM;
+  uint32_t seen = 0;
+  for (;;) {
+    const uint32_t sub =
+        __shfl(lane == 0 ? __hip_atomic_load(a.submitted, __ATOMIC_ACQUIRE, sys) : 0u, 0, 64);
+    if (sub != seen) {
+      for (uint32_t i = seen + lane; i < sub; i += kWave)
+        for (int w = 0; w < 3; ++w)
+          st_rlx(a.dev_inbox + (long)i * 4 + w,
+                 __hip_atomic_load(a.inbox + (long)i * 4 + w, __ATOMIC_RELAXED, sys));
+      __syncthreads();
+      if (lane == 0) st_rel(a.dev_ctl, sub);
+      seen = sub;
+    }
+    int stop = 0;
+    if (lane == 0) {
+      stop = __hip_atomic_load(a.stop, __ATOMIC_ACQUIRE, sys) != 0;
+      if (stop) st_rel(a.dev_ctl + 32, 1u);
+      if (!g.ok()) stop = 1;
+      if (!stop) sleep_quanta(16);
+    }
+    if (__shfl(stop, 0, 64)) return;
+  }
+}
+
 template <int TREE, bool SERVE>
 __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
   constexpr int W = Rec<TREE>::W;
@@ -234,6 +282,10 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
   uint32_t *deq = a.deq + (long)blockIdx.x * (dq_cap * W + kDqSkew);
   Guard g;
   g.init(a.ctl, a.ticks);
+  if (SERVE && blockIdx.x == gridDim.x - 1) {  // the dispatcher, not a worker
+    dispatch(a, g);
+    return;
+  }
   for (int s = lane; s < kMaxSlots; s += kWave) {
     s_hold[s] = 0;
     s_cred[s] = 0;
@@ -563,19 +615,19 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
         continue;
       }
     }
-    if (SERVE) {  // admit the next submitted request, if any
+    // admit the next submitted request, if any.  The request counter lives
+    // in host memory: each idle WG looks every 4th pass, staggered
+    if (SERVE && ((idle_iter + (uint32_t)me) & 3) == 0) {
       __shared__ int s_adm;
       if (lane == 0) {
         s_adm = -1;
-        const uint32_t sub = __hip_atomic_load(a.submitted, __ATOMIC_ACQUIRE,
-                                               __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint32_t sub = ld_acq(a.dev_ctl);
         const uint32_t adm = ld_rlx(a.admitted);
         if (adm < sub && cas_rlx(a.admitted, adm, adm + 1) == adm) {
-          const uint32_t *q = a.inbox + (long)adm * 4;
-          const int sys = __HIP_MEMORY_SCOPE_SYSTEM;
-          const uint32_t slot = __hip_atomic_load(q + 0, __ATOMIC_RELAXED, sys);
-          const uint32_t seed = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, sys);
-          const uint32_t gmx = __hip_atomic_load(q + 2, __ATOMIC_RELAXED, sys);
+          const uint32_t *q = a.dev_inbox + (long)adm * 4;
+          const uint32_t slot = ld_rlx(q + 0);
+          const uint32_t seed = ld_rlx(q + 1);
+          const uint32_t gmx = ld_rlx(q + 2);
           __hip_atomic_store(a.nodes + slot, 0ull, __ATOMIC_RELAXED, HS_AGENT);
           __hip_atomic_store(a.leaves + slot, 0ull, __ATOMIC_RELAXED, HS_AGENT);
           __hip_atomic_store(a.leafsum + slot, 0ull, __ATOMIC_RELAXED, HS_AGENT);
@@ -624,10 +676,9 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
     if (lane == 0) {
       // the termination words are shared by every idle WG: look every 8th pass
       bool fin = false;
-      if ((idle_iter++ & 7) == 0) {
+      if ((idle_iter & 7) == 0) {
         if (SERVE)
-          fin = __hip_atomic_load(a.stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != 0 &&
-                ld_rlx(a.live) == 0;
+          fin = ld_acq(a.dev_ctl + 32) != 0 && ld_rlx(a.live) == 0;
         else
           fin = ld_rlx(a.credit) == 0;
       }
@@ -638,6 +689,7 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
     __syncthreads();
     tick(false);
     idle_q = idle_q >= 32 ? 32 : 2 * idle_q;
+    ++idle_iter;
     if (s_exit) {
       flush_stats();
       return;
@@ -681,16 +733,32 @@ int geo_thresholds(double b0, std::vector<uint32_t> &thr) {
   return (int)thr.size();
 }
 
+// Device state of one engine.  The per-run words sit in one block that a
+// single fill clears: inbox counts, request rings, credits, the results span
+// (nodes, leaves, leaf sums, max depths, control words: copied back in one
+// piece), depth limits and the serving counters.
 struct Engine {
   DevArena ar;
   UtsArgs a;
-  int blocks = 0, W = 0;
+  int tree = -1, blocks = 0, W = 0, dq_log2 = 0;
+  double b0 = 0;
   uint32_t *d_thr = nullptr;
+  uint32_t *state = nullptr;
+  size_t state_words = 0;
+  uint32_t *results = nullptr;  // kResultWords words inside `state`
+  static constexpr size_t kResultWords = 3 * 2 * kMaxSlots + kMaxSlots + 64;
+
+  bool matches(int t, int nb, int dq, double b) const {
+    return tree == t && blocks == nb && dq_log2 == dq && (t == 0 || b0 == b);
+  }
 
   // tree: 0 reference, 1 geometric
-  int setup(hs_ctx *c, int tree, int num_blocks, int dq_log2, double b0) {
+  int setup(hs_ctx *c, int tree_, int num_blocks, int dq_log2_, double b0_) {
     std::memset(&a, 0, sizeof a);
-    blocks = num_blocks > 0 ? num_blocks : 4 * c->num_cu;
+    tree = tree_;
+    dq_log2 = dq_log2_;
+    b0 = b0_;
+    blocks = num_blocks > 0 ? num_blocks : default_blocks(c);
     W = tree == 1 ? 6 : 2;
     a.dq_log2 = dq_log2;
     // one inbox holds at most a quarter deque, so a late delivery fits
@@ -704,33 +772,34 @@ struct Engine {
     uint32_t ring = 1;
     while (ring < 2u * per_ring) ring <<= 1;
     a.rq_mask = ring - 1;
+    // state block layout, every region on a 256-byte boundary
+    auto up = [](size_t w) { return (w + 63) & ~(size_t)63; };
+    const size_t o_wbn = 0;
+    const size_t o_rq = up(o_wbn + (size_t)blocks * HS_LINE_WORDS);
+    const size_t o_ring = up(o_rq + (size_t)a.nrq * HS_LINE_WORDS);
+    const size_t o_cred = up(o_ring + (size_t)a.nrq * ring);
+    const size_t o_res = up(o_cred + (size_t)kMaxSlots * HS_LINE_WORDS);
+    const size_t o_gmx = up(o_res + kResultWords);
+    const size_t o_live = up(o_gmx + kMaxSlots);
+    state_words = up(o_live + 64);
     int rc;
     if ((rc = ar.alloc(&a.deq, (size_t)blocks * (((size_t)W << dq_log2) + kDqSkew)))) return rc;
     if ((rc = ar.alloc(&a.wbox, (size_t)blocks * a.wbox_cap * W))) return rc;
-    if ((rc = ar.alloc(&a.wbox_n, (size_t)blocks * HS_LINE_WORDS))) return rc;
-    if ((rc = ar.alloc(&a.rq, (size_t)a.nrq * HS_LINE_WORDS))) return rc;
-    if ((rc = ar.alloc(&a.rq_ring, (size_t)a.nrq * ring))) return rc;
-    if ((rc = ar.alloc(&a.credit, (size_t)kMaxSlots * HS_LINE_WORDS))) return rc;
-    if ((rc = ar.alloc(&a.nodes, kMaxSlots))) return rc;
-    if ((rc = ar.alloc(&a.leaves, kMaxSlots))) return rc;
-    if ((rc = ar.alloc(&a.leafsum, kMaxSlots))) return rc;
-    if ((rc = ar.alloc(&a.maxd, kMaxSlots))) return rc;
-    if ((rc = ar.alloc(&a.slot_gen_mx, kMaxSlots))) return rc;
+    if ((rc = ar.alloc(&state, state_words))) return rc;
     if ((rc = ar.alloc(&d_thr, kMaxThr))) return rc;
-    if ((rc = ar.alloc(&a.ctl, 64))) return rc;
-    if ((rc = ar.alloc(&a.live, 64))) return rc;
+    a.wbox_n = state + o_wbn;
+    a.rq = state + o_rq;
+    a.rq_ring = state + o_ring;
+    a.credit = state + o_cred;
+    results = state + o_res;
+    a.nodes = reinterpret_cast<unsigned long long *>(results);
+    a.leaves = a.nodes + kMaxSlots;
+    a.leafsum = a.leaves + kMaxSlots;
+    a.maxd = results + 6 * kMaxSlots;
+    a.ctl = a.maxd + kMaxSlots;
+    a.slot_gen_mx = state + o_gmx;
+    a.live = state + o_live;
     a.admitted = a.live + 32;
-    HS_HIP(hipMemset(a.wbox_n, 0, (size_t)blocks * HS_LINE_WORDS * 4));
-    HS_HIP(hipMemset(a.rq, 0, (size_t)a.nrq * HS_LINE_WORDS * 4));
-    HS_HIP(hipMemset(a.rq_ring, 0, (size_t)a.nrq * ring * 4));
-    HS_HIP(hipMemset(a.credit, 0, (size_t)kMaxSlots * HS_LINE_WORDS * 4));
-    HS_HIP(hipMemset(a.nodes, 0, kMaxSlots * 8));
-    HS_HIP(hipMemset(a.leaves, 0, kMaxSlots * 8));
-    HS_HIP(hipMemset(a.leafsum, 0, kMaxSlots * 8));
-    HS_HIP(hipMemset(a.maxd, 0, kMaxSlots * 4));
-    HS_HIP(hipMemset(a.slot_gen_mx, 0, kMaxSlots * 4));
-    HS_HIP(hipMemset(a.ctl, 0, 256));
-    HS_HIP(hipMemset(a.live, 0, 256));
     if (tree == 1) {
       std::vector<uint32_t> thr;
       a.nthr = geo_thresholds(b0, thr);
@@ -739,45 +808,68 @@ struct Engine {
     }
     a.thr = d_thr;
     a.ticks = timeout_ticks(c, 0);
+    HS_HIP(hipMemset(state, 0, state_words * 4));
     return HS_OK;
   }
 
-  // Delivers one root record (W words) with its credit to WG 0's inbox.
-  int seed_root(const uint32_t *rec, int slot) {
-    for (int w = 0; w < W; ++w)  // record 0 of the inbox: word w at w * 64
-      HS_HIP(hipMemcpy(a.wbox + (size_t)w * 64, rec + w, 4, hipMemcpyHostToDevice));
-    const uint32_t one = 1;
-    HS_HIP(hipMemcpy(a.wbox_n, &one, 4, hipMemcpyHostToDevice));
-    HS_HIP(hipMemcpy(a.credit + (size_t)slot * HS_LINE_WORDS, &one, 4, hipMemcpyHostToDevice));
+  // Clears the per-run state on the stream (a fill kernel).
+  int reset(hipStream_t s) {
+    HS_HIP(hipMemsetAsync(state, 0, state_words * 4, s));
     return HS_OK;
   }
 
-  int read_slot(int slot, hs_uts_result *r) {
-    unsigned long long n = 0, l = 0, ls = 0;
-    uint32_t md = 0, ctl[64];
-    HS_HIP(hipMemcpy(&n, a.nodes + slot, 8, hipMemcpyDeviceToHost));
-    HS_HIP(hipMemcpy(&l, a.leaves + slot, 8, hipMemcpyDeviceToHost));
-    HS_HIP(hipMemcpy(&ls, a.leafsum + slot, 8, hipMemcpyDeviceToHost));
-    HS_HIP(hipMemcpy(&md, a.maxd + slot, 4, hipMemcpyDeviceToHost));
-    HS_HIP(hipMemcpy(ctl, a.ctl, sizeof ctl, hipMemcpyDeviceToHost));
-    r->nodes = n;
-    r->leaves = l;
-    r->leaf_sum = ls;
-    r->max_depth = (int)md;
+  // Copies the results span back and fills r for one slot.
+  int read_slot(hipStream_t s, int slot, hs_uts_result *r) {
+    uint32_t res[kResultWords];
+    HS_HIP(hipMemcpyAsync(res, results, sizeof res, hipMemcpyDeviceToHost, s));
+    HS_HIP(hipStreamSynchronize(s));
+    auto u64 = [&](int k) { return (unsigned long long)res[2 * k] | ((unsigned long long)res[2 * k + 1] << 32); };
+    const uint32_t *ctl = res + 7 * kMaxSlots;
+    r->nodes = u64(slot);
+    r->leaves = u64(kMaxSlots + slot);
+    r->leaf_sum = u64(2 * kMaxSlots + slot);
+    r->max_depth = (int)res[6 * kMaxSlots + slot];
     r->aborted = ctl[0] != 0;
     if (ctl[32]) return HS_ENOMEM;  // a deque ring overflowed
     return ctl[0] ? HS_ETIMEOUT : HS_OK;
   }
+
+  static int default_blocks(hs_ctx *c) { return 8 * c->num_cu; }
 };
 
+// Root of a run, delivered to WG 0's inbox with its credit: the geometric
+// root digest SHA1(16 zero bytes || seed) at depth 0, or node 0 of the
+// reference tree.  One wave.
+__global__ void k_seed_root(UtsArgs a, int tree, uint32_t seed, uint32_t gen_mx) {
+  if (threadIdx.x != 0) return;
+  if (tree == 1) {
+    uint32_t st[5];
+    uts_root(seed, st);
+    for (int w = 0; w < 5; ++w) a.wbox[w * 64] = st[w];  // record 0: word w at w * 64
+    a.wbox[5 * 64] = 0;                                  // depth 0, slot 0
+  } else {
+    a.wbox[0] = 0;   // node 0
+    a.wbox[64] = 0;  // slot 0
+  }
+  a.slot_gen_mx[0] = gen_mx;
+  a.credit[0] = 1;
+  a.wbox_n[0] = 1;
+}
+
 // HS_UTS_STATS=1 in the environment: per-WG statistics of the next run,
-// summarised on stderr (busy/idle time, batch fill, donations, steals).
-int stats_begin(Engine &e) {
+// summarised on stderr (busy/idle time, batch fill, hand-offs).
+int stats_begin(Engine &e, hipStream_t s) {
   const char *v = std::getenv("HS_UTS_STATS");
-  if (!v || !*v || *v == '0') return HS_OK;
-  int rc;
-  if ((rc = e.ar.alloc(&e.a.stats, (size_t)e.blocks * kStatWords))) return rc;
-  HS_HIP(hipMemset(e.a.stats, 0, (size_t)e.blocks * kStatWords * 8));
+  if (!v || !*v || *v == '0') {
+    e.a.stats = nullptr;
+    return HS_OK;
+  }
+  static_assert(sizeof(unsigned long long) == 8, "stats words");
+  if (!e.a.stats) {
+    int rc;
+    if ((rc = e.ar.alloc(&e.a.stats, (size_t)e.blocks * kStatWords))) return rc;
+  }
+  HS_HIP(hipMemsetAsync(e.a.stats, 0, (size_t)e.blocks * kStatWords * 8, s));
   return HS_OK;
 }
 
@@ -808,14 +900,17 @@ int stats_report(Engine &e, double kernel_ms) {
   return HS_OK;
 }
 
+// One run: clear the state, deliver the root, run the kernel, read slot 0.
 template <int TREE>
-int run_single(hs_ctx *c, Engine &e, hs_uts_result *res) {
-  int src = stats_begin(e);
-  if (src) return src;
+int run_single(hs_ctx *c, Engine &e, uint32_t seed, uint32_t gen_mx, hs_uts_result *res) {
+  int rc;
+  if ((rc = e.reset(c->stream))) return rc;
+  if ((rc = stats_begin(e, c->stream))) return rc;
+  hipLaunchKernelGGL(k_seed_root, dim3(1), dim3(kWave), 0, c->stream, e.a, TREE, seed, gen_mx);
+  HS_HIP(hipGetLastError());
   hipEvent_t e0, e1;
   HS_HIP(hipEventCreate(&e0));
   HS_HIP(hipEventCreate(&e1));
-  HS_HIP(hipDeviceSynchronize());
   HS_HIP(hipEventRecord(e0, c->stream));
   hipLaunchKernelGGL((k_uts<TREE, false>), dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
   HS_HIP(hipGetLastError());
@@ -826,9 +921,11 @@ int run_single(hs_ctx *c, Engine &e, hs_uts_result *res) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   res->kernel_ms = ms;
-  if ((src = stats_report(e, ms))) return src;
-  return e.read_slot(0, res);
+  if ((rc = stats_report(e, ms))) return rc;
+  return e.read_slot(c->stream, 0, res);
 }
+
+void free_engine(void *p) { delete static_cast<Engine *>(p); }
 
 }  // namespace
 
@@ -857,35 +954,34 @@ extern "C" int hs_uts_ref_run(hs_ctx *c, int num_nodes, const uint32_t *value,
   e.a.value = d_val;
   e.a.child_a = d_a;
   e.a.child_b = d_b;
-  const uint32_t root[2] = {0, 0};
-  if ((rc = e.seed_root(root, 0))) return rc;
-  return run_single<0>(c, e, res);
+  return run_single<0>(c, e, 0, 0, res);
 }
 
+// The geometric engine of a context is kept between calls (same grid, deque
+// size and b0), so a call costs a state fill, the root delivery, the kernel
+// and one small copy back.
 extern "C" int hs_uts_geo_run(hs_ctx *c, int seed, double b0, int gen_mx, int num_blocks,
                               hs_uts_result *res) {
   if (!c || !res || !(b0 > 0 && b0 < 1000) || gen_mx < 0 || gen_mx > 60000 || num_blocks < 0)
     return HS_EINVAL;
   std::memset(res, 0, sizeof *res);
   HS_HIP(hipSetDevice(c->device));
-  Engine e;
-  int rc = e.setup(c, 1, num_blocks, 16, b0);
-  if (rc) return rc;
-  const uint32_t g = (uint32_t)gen_mx;
-  HS_HIP(hipMemcpy(e.a.slot_gen_mx, &g, 4, hipMemcpyHostToDevice));
-  // root digest: SHA1(16 zero bytes || seed) computed on the device
-  uint32_t root[6];
-  {
-    uint32_t *d_root;
-    if ((rc = e.ar.alloc(&d_root, 8))) return rc;
-    hipLaunchKernelGGL(k_root_digest, dim3(1), dim3(1), 0, c->stream, (uint32_t)seed, d_root);
-    HS_HIP(hipGetLastError());
-    HS_HIP(hipMemcpyAsync(root, d_root, 20, hipMemcpyDeviceToHost, c->stream));
-    HS_HIP(hipStreamSynchronize(c->stream));
+  const int nb = num_blocks > 0 ? num_blocks : Engine::default_blocks(c);
+  Engine *e = static_cast<Engine *>(c->uts_cache);
+  if (!e || !e->matches(1, nb, 16, b0)) {
+    if (c->uts_cache_free) c->uts_cache_free(c->uts_cache);
+    c->uts_cache = nullptr;
+    c->uts_cache_free = nullptr;
+    e = new Engine;
+    int rc = e->setup(c, 1, nb, 16, b0);
+    if (rc) {
+      delete e;
+      return rc;
+    }
+    c->uts_cache = e;
+    c->uts_cache_free = free_engine;
   }
-  root[5] = 0;  // depth 0, slot 0
-  if ((rc = e.seed_root(root, 0))) return rc;
-  return run_single<1>(c, e, res);
+  return run_single<1>(c, *e, (uint32_t)seed, (uint32_t)gen_mx, res);
 }
 
 extern "C" int hs_uts_serve(hs_ctx *c, const hs_uts_request *req, int n, const hs_serve_cfg *cfg,
@@ -919,6 +1015,13 @@ extern "C" int hs_uts_serve(hs_ctx *c, const hs_uts_request *req, int n, const h
   e.a.inbox = h_inbox;
   e.a.submitted = h_ctl;
   e.a.stop = h_ctl + 32;
+  // device copies of the requests and control words (one dispatcher WG,
+  // the grid's last, keeps them; the others are workers)
+  if (e.blocks < 2) return HS_EINVAL;
+  if ((rc = e.ar.alloc(&e.a.dev_inbox, (size_t)n * 4))) return rc;
+  if ((rc = e.ar.alloc(&e.a.dev_ctl, 64))) return rc;
+  HS_HIP(hipMemset(e.a.dev_inbox, 0, (size_t)n * 16));
+  HS_HIP(hipMemset(e.a.dev_ctl, 0, 256));
   e.a.resp = h_resp;
 
   using clk = std::chrono::steady_clock;
@@ -974,7 +1077,7 @@ extern "C" int hs_uts_serve(hs_ctx *c, const hs_uts_request *req, int n, const h
       if (done < n) rc = HS_ETIMEOUT;
       break;
     }
-    std::this_thread::sleep_for(std::chrono::microseconds(20));
+    std::this_thread::sleep_for(std::chrono::microseconds(5));
   }
   __atomic_store_n(stop, 1u, __ATOMIC_RELEASE);
   HS_HIP(hipStreamSynchronize(c->stream));

@@ -4,8 +4,8 @@
   C2  grid barriers 1024 WGs x 256 lanes, numLdSt=10, numCSIters=1000
   C3  64-instance mutex/semaphore mix: batched persistent launch vs plain
   C4  relaxed atomics 1024 WGs x 256 lanes x 10^4 iterations
-  C5  UTS serving loop (Poisson arrivals, geometric SHA-1 trees) vs one
-      launch per tree
+  C5  UTS serving loop vs one launch per tree: saturation throughput and
+      latency under Poisson arrivals, geometric SHA-1 trees
 
 Every measurement is parity-checked (oracle golden / closed form / exact
 counts) and reported next to its unit rate.  Usage:
@@ -132,42 +132,86 @@ def c4(dev, oh, iters=10000):
     return out
 
 
-def c5(dev, oh, n=32, seed=9, load=0.8, slots=16):
-    """Poisson arrivals of geometric (b0=4) trees, depth ~ U[6, 14]."""
+def c5(dev, oh, n=256, seed=9, load=0.8, slots=16, depth=(6, 11)):
+    """UTS serving vs one launch per tree, geometric trees (b0 = 4) with
+    depth limits drawn from `depth` (about 4K to 5M nodes each).
+
+    saturation: every request present at t = 0; serving nodes/s (wall clock
+      of the whole session) against the plain calls back to back, both by
+      wall clock (engine kept between calls) and by kernel time alone.
+    poisson: arrivals at `load` x the plain kernel capacity; latency =
+      completion - arrival.  Plain latencies come from a FIFO queue served by
+      the measured per-call wall times (and, as the plain side's best case,
+      by the kernel times alone).
+    parity: every tree's (nodes, leaves, max depth) is the same served in the
+      mix as launched alone."""
     rng = np.random.default_rng(seed)
-    trees = [(int(rng.integers(0, 1 << 30)), int(rng.integers(6, 15))) for _ in range(n)]
-    # plain: one launch per tree, back to back (also gives the capacity)
+    trees = [(int(rng.integers(0, 1 << 30)), int(rng.integers(depth[0], depth[1] + 1)))
+             for _ in range(n)]
+    dev.uts_geo(1, 4.0, 4)  # builds the context's engine once
     plain = []
     t0 = time.perf_counter()
     for sd, d in trees:
+        t1 = time.perf_counter()
         r = dev.uts_geo(sd, 4.0, d)
-        plain.append((r.nodes, r.leaves, r.max_depth, r.kernel_ms))
+        plain.append((r.nodes, r.leaves, r.max_depth, r.kernel_ms / 1e3,
+                      time.perf_counter() - t1))
     plain_wall = time.perf_counter() - t0
     total_nodes = sum(p[0] for p in plain)
-    capacity = total_nodes / sum(p[3] / 1e3 for p in plain)  # nodes/s of the engine
-    mean_nodes = total_nodes / n
-    lam = load * capacity / mean_nodes  # trees per second
+    plain_kernel_s = sum(p[3] for p in plain)
+    capacity = total_nodes / plain_kernel_s
+
+    def counts_ok(resp):
+        return all((r.nodes, r.leaves, r.max_depth) == p[:3] for r, p in zip(resp, plain))
+
+    def pct(v, q):
+        v = np.sort(np.asarray(v))
+        return float(v[int(q * (len(v) - 1) + 0.5)])
+
+    def split(resp, arr):
+        wait = [r.admit_s - a for r, a in zip(resp, arr)]
+        svc = [r.done_s - r.admit_s for r in resp]
+        return {"admit_wait_p50_s": pct(wait, 0.5), "admit_wait_p99_s": pct(wait, 0.99),
+                "service_p50_s": pct(svc, 0.5), "service_p99_s": pct(svc, 0.99)}
+
+    # saturation
+    more = {}
+    for sl in sorted({slots, 64}):
+        resp, summ = dev.uts_serve([(sd, d, 4.0, 0.0) for sd, d in trees], slots=sl,
+                                   timeout_s=120.0)
+        more[f"slots_{sl}_nodes_per_s"] = summ.total_nodes / summ.wall_s
+    resp, summ = dev.uts_serve([(sd, d, 4.0, 0.0) for sd, d in trees], slots=slots,
+                               timeout_s=120.0)
+    sat = {"serving_nodes_per_s": summ.total_nodes / summ.wall_s, "by_slots": more,
+           "serving_wall_s": summ.wall_s,
+           "plain_nodes_per_s_kernel": capacity, "plain_kernel_s": plain_kernel_s,
+           "plain_nodes_per_s_wall": total_nodes / plain_wall, "plain_wall_s": plain_wall,
+           "speedup_vs_kernel": (summ.total_nodes / summ.wall_s) / capacity,
+           "speedup_vs_wall": (summ.total_nodes / summ.wall_s) / (total_nodes / plain_wall)}
+    ok = counts_ok(resp)
+    # Poisson arrivals
+    lam = load * capacity / (total_nodes / n)
     arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
     arrivals -= arrivals[0]
-    reqs = [(sd, d, 4.0, float(a)) for (sd, d), a in zip(trees, arrivals)]
-    resp, summ = dev.uts_serve(reqs, slots=slots, timeout_s=120.0)
-    ok = all((r.nodes, r.leaves, r.max_depth) == p[:3] for r, p in zip(resp, plain))
-    # plain under the same arrivals: FIFO, one tree at a time
-    t = 0.0
-    lat = []
-    for a, p in zip(arrivals, plain):
-        t = max(t, a) + p[3] / 1e3
-        lat.append(t - a)
-    lat = np.sort(lat)
-    return {"trees": n, "total_nodes": int(total_nodes), "offered_load": load,
-            "serving": {"nodes_per_s": summ.total_nodes / summ.wall_s, "wall_s": summ.wall_s,
-                        "p50_s": summ.p50_s, "p99_s": summ.p99_s},
-            "plain_one_launch_per_tree": {
-                "nodes_per_s": total_nodes / (sum(p[3] for p in plain) / 1e3),
-                "wall_s_back_to_back": plain_wall,
-                "p50_s": float(lat[int(0.5 * (n - 1) + 0.5)]),
-                "p99_s": float(lat[int(0.99 * (n - 1) + 0.5)])},
-            "parity_ok": ok}
+    resp, summ = dev.uts_serve([(sd, d, 4.0, float(a)) for (sd, d), a in zip(trees, arrivals)],
+                               slots=slots, timeout_s=120.0)
+    ok = ok and counts_ok(resp)
+    def fifo(col):
+        t, lat = 0.0, []
+        for a, p in zip(arrivals, plain):
+            t = max(t, a) + p[col]
+            lat.append(t - a)
+        lat = np.sort(lat)
+        return {"p50_s": float(lat[int(0.5 * (n - 1) + 0.5)]),
+                "p99_s": float(lat[int(0.99 * (n - 1) + 0.5)])}
+
+    pois = {"offered_load": load, "arrival_rate_per_s": lam,
+            "serving": {"p50_s": summ.p50_s, "p99_s": summ.p99_s,
+                        "nodes_per_s": summ.total_nodes / summ.wall_s,
+                        **split(resp, arrivals)},
+            "plain_fifo_kernel": fifo(3), "plain_fifo_wall": fifo(4)}
+    return {"trees": n, "depths": list(depth), "total_nodes": int(total_nodes), "slots": slots,
+            "saturation": sat, "poisson": pois, "parity_ok": ok}
 
 
 def main(argv):
