@@ -265,7 +265,7 @@ __global__ void k_init_locks(uint32_t *locks, int nlocks, uint32_t sem_max,
 
 bool is_barrier(int p) { return p < 4 || (p >= 37 && p <= 42); }
 bool is_mutex(int p) { return (p >= 4 && p <= 7) || (p >= 20 && p <= 23); }
-bool is_sem(int p) { return p >= 8 && p <= 35; }
+bool is_sem(int p) { return (p >= 8 && p <= 19) || (p >= 24 && p <= 35); }
 // per-group locks and regions: the uniq mutexes and uniq semaphores.  The
 // priority EBO semaphores 32-35 are dispatched to the global kernel
 // (main.hip.cpp:2455-2470 -> invokeEBOSemaphorePriority) although their
