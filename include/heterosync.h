@@ -45,8 +45,9 @@ enum {
   HS_EOCCUPANCY = -4,  /* workgroups cannot all be co-resident */
   HS_ETIMEOUT = -5,    /* device watchdog fired: a wait exceeded timeout_s */
   HS_ENODEV = -6,      /* no HIP device */
-  HS_EUNSUPPORTED = -7 /* primitive id exists in the reference table but is
-                          not part of this build (SRB, priority semaphores) */
+  HS_EUNSUPPORTED = -7 /* primitive id without a kernel in this build (every
+                          id of the reference table has one; kept for ABI
+                          stability) */
 };
 
 const char *hs_strerror(int code);
@@ -68,7 +69,8 @@ int hs_device_info(hs_ctx *ctx, int *num_cu, int *max_threads_per_block,
  * 43-46 reserved).  Returns -1 for unknown names. */
 int hs_prim_id(const char *name);
 const char *hs_prim_name(int id);
-/* 1 if this build implements the id (0-15, 20-31), else 0. */
+/* 1 if this build implements the id (0-35 and 37-42; 36 is reserved in the
+ * reference table), else 0. */
 int hs_prim_supported(int id);
 
 /* Number of int32 storage words a run uses (main.hip.cpp:2109-2136 with
