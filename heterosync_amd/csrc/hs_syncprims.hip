@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(1024) k_mutex(SPArgs a) {
     if (is_master()) s_ok = mutex_lock<KIND>(lk, ring, a.ring_stride, ticket, a.bo, g);
     __syncthreads();
     if (!s_ok) return;
-    wg_access<MODE>(a.storage, tid0, a.L);
+    wg_move<MODE>(a.storage, tid0, a.L);
     __syncthreads();
     if (is_master()) {
       st_rlx(lk + kLkCs, ld_rlx(lk + kLkCs) + 1);  // plain, under the lock
@@ -91,12 +91,12 @@ __global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
       if (UNIQ) {  // main.hip.cpp:1040-1058: own region, then +numCU WGs
         int cur = b;
         for (int j = 0; j < per; ++j) {
-          wg_access<MODE>(a.storage, (long)cur * blockDim.x, a.L);
+          wg_move<MODE>(a.storage, (long)cur * blockDim.x, a.L);
           cur = (cur + nab) % grid;
         }
       } else {  // main.hip.cpp:903-915: regions 0..perCU-1
         for (int j = 0; j < per; ++j)
-          wg_access<MODE>(a.storage, (long)j * blockDim.x, a.L);
+          wg_move<MODE>(a.storage, (long)j * blockDim.x, a.L);
       }
     } else {
       wg_read(a.storage, (long)(UNIQ ? b : pw) * blockDim.x, a.L, sink);
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
   __syncthreads();
   if (!s_ok) return;
   for (int i = 0; i < a.iters; ++i) {
-    wg_access<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
+    wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
     if (LX && a.per_cu > 1) {
       bool ok;
       if (FLAG) {
@@ -150,14 +150,14 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
         if (XCD) wait_mem();
         __syncthreads();
         if (is_master())
-          s_ok = XCD ? xcd_tree_arrive(a.xt, grp, true, a.bo.poll_q, g)
+          s_ok = XCD ? xcd_tree_arrive(a.xt, grp, true, a.e_global0 + i + 1, a.bo.poll_q, g)
                      : atomic_tree_arrive(a.tb, grp, true, a.bo.poll_q, g);
         __syncthreads();
         ok = s_ok;
       }
       if (!ok) return;
       cur = (cur + nab) % grid;  // HM:355-358
-      wg_access<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
+      wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
     }
     bool ok;
     if (FLAG) {
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
       if (XCD) wait_mem();
       __syncthreads();
       if (is_master())
-        s_ok = XCD ? xcd_tree_arrive(a.xt, grp, false, a.bo.poll_q, g)
+        s_ok = XCD ? xcd_tree_arrive(a.xt, grp, false, a.e_global0 + i + 1, a.bo.poll_q, g)
                    : atomic_tree_arrive(a.tb, grp, false, a.bo.poll_q, g);
       __syncthreads();
       ok = s_ok;
@@ -219,13 +219,14 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
   }
   __syncthreads();
   if (!s_ok) return;
+  uint32_t epoch = a.e_global0;  // barriers of each kind completed (tree form)
   auto barrier = [&](bool local) -> bool {
     wait_mem();
     __syncthreads();
     bool ok = true;
     if ((local && VAR == 1) || (!local && tree)) {
       if (is_master())
-        ok = XCD ? xcd_tree_arrive(a.xt, grp, local, a.bo.poll_q, g)
+        ok = XCD ? xcd_tree_arrive(a.xt, grp, local, epoch, a.bo.poll_q, g)
                  : atomic_tree_arrive(a.tb, grp, local, a.bo.poll_q, g);
     } else {
       uint32_t *blk = a.flat + (local ? kBarWords : 0);
@@ -236,11 +237,12 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
     return __syncthreads_and(ok) != 0;
   };
   for (int i = 0; i < a.iters; ++i) {
-    wg_access<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
+    epoch = a.e_global0 + i + 1;
+    wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
     if (lx) {
       if (!barrier(true)) return;
       if (VAR != 3) cur = (cur + nab) % grid;
-      wg_access<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
+      wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
     }
     if (!barrier(false)) return;
     cur = (cur + 1) % grid;
@@ -569,25 +571,38 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
       std::memcpy(res->lock_words, w, sizeof(w));
     } else if (p <= 1 || srb) {
       const size_t g0 = xcd_mode ? kBarWords * (1 + nxcd) : kBarWords;
-      ok = ok && bar[0] == 0 && bar[32] == te;
-      if (xcd_mode)
-        for (int x = 0; x < std::min(nab, nxcd); ++x)
-          ok = ok && bar[kBarWords * (1 + x)] == 0 &&
-               bar[kBarWords * (1 + nxcd + nab + x)] == te;  // release copy
-      if (per > 1)
-        for (int gi = 0; gi < nab; ++gi)
-          ok = ok && bar[g0 + kBarWords * gi] == 0 &&
-               bar[g0 + kBarWords * gi + 32] == (xcd_mode ? (lx ? te : 0u) : tl);
+      if (xcd_mode) {
+        // counters are never reset: each ends at episodes x level size
+        const int nx = std::min(nab, nxcd);
+        ok = ok && bar[0] == te * (uint32_t)nx && bar[32] == te;
+        for (int x = 0; x < nx; ++x) {
+          const uint32_t ngx = (uint32_t)((nab - x + nxcd - 1) / nxcd);
+          ok = ok && bar[kBarWords * (1 + x)] == te * ngx &&
+               bar[kBarWords * (1 + nxcd + nab + x)] == te;  // release word
+        }
+        if (per > 1)
+          for (int gi = 0; gi < nab; ++gi)
+            ok = ok && bar[g0 + kBarWords * gi] == te * (uint32_t)per * (lx ? 2u : 1u) &&
+                 bar[g0 + kBarWords * gi + 32] == (lx ? te : 0u);
+      } else {
+        ok = ok && bar[0] == 0 && bar[32] == te;
+        if (per > 1)
+          for (int gi = 0; gi < nab; ++gi)
+            ok = ok && bar[g0 + kBarWords * gi] == 0 && bar[g0 + kBarWords * gi + 32] == tl;
+      }
       const uint32_t w[8] = {bar[0], bar[32], bar[g0], bar[g0 + 32], ep, 0, 0, 0};
       std::memcpy(res->lock_words, w, sizeof(w));
     } else {
       const uint32_t *f = &flags[4 * HS_LINE_WORDS];
       const uint32_t *lin, *lout;
-      if (xcd_mode) {
+      uint32_t lout_want = tl;
+      if (xcd_mode) {  // single-hop release words gout[x]; out-flags carry local epochs only
         const uint32_t *gin = f, *gout = f + nxcd, *xin = f + 2 * nxcd, *xout = xin + nab;
-        for (int x = 1; x < std::min(nab, nxcd); ++x) ok = ok && gin[x] == te && gout[x] == te;
-        for (int gi = nxcd; gi < nab; ++gi) ok = ok && xin[gi] == te && xout[gi] == te;
+        const int nx = std::min(nab, nxcd);
+        for (int x = 0; x < nx; ++x) ok = ok && gout[x] == te && (x == 0 || gin[x] == te);
+        for (int gi = nxcd; gi < nab; ++gi) ok = ok && xin[gi] == te && xout[gi] == 0;
         lin = xout + nab;
+        lout_want = lx ? tl - 1 : 0;
       } else {
         const uint32_t *gin = f, *gout = f + nab;
         for (int gi = 1; gi < nab; ++gi) ok = ok && gin[gi] == te && gout[gi] == te;
@@ -597,7 +612,7 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
       if (per > 1)
         for (int gi = 0; gi < nab; ++gi)
           for (int m = 1; m < per; ++m)
-            ok = ok && lin[(size_t)gi * per + m] == tl && lout[(size_t)gi * per + m] == tl;
+            ok = ok && lin[(size_t)gi * per + m] == tl && lout[(size_t)gi * per + m] == lout_want;
       const uint32_t w[8] = {f[1], f[1 + (xcd_mode ? nxcd : nab)], per > 1 ? lin[1] : 0,
                              per > 1 ? lout[1] : 0, ep, 0, 0, 0};
       std::memcpy(res->lock_words, w, sizeof(w));

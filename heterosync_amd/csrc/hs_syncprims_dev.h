@@ -33,6 +33,9 @@ constexpr int kLkPrio = 48;
 // Barrier block: 64 words per level instance (count @0, generation @32).
 constexpr int kBarWords = 64;
 
+// Waits until every memory access this wave issued has completed.
+__device__ __forceinline__ void wait_mem() { __builtin_amdgcn_s_waitcnt(0); }
+
 enum MutexKind { kSpin = 0, kEBO = 1, kSleep = 2, kFA = 3 };
 
 struct Backoff {
@@ -106,6 +109,52 @@ __device__ __forceinline__ void wg_access(int32_t *s, long tid0, int L) {
   const long tid = tid0 + threadIdx.x;
   access_data<MODE, SC1>(s, (tid / kWordsPerLine) * (L + 1),
                          threadIdx.x % kWordsPerLine, L);
+}
+
+// The same pass done by the whole WG with 16-byte accesses.  A WG's region
+// is the contiguous chains of its blockDim threads (blockDim is a multiple
+// of 64, so the region starts on a chain), and each chain is L+1 lines of
+// 64 B.  The WG moves it band by band, top band first (at most 16 source
+// lines, i.e. 4 pieces per thread), with all loads of a band complete before
+// any of its stores -- exactly the per-thread top-down walk's result, since
+// a band's stores only overwrite lines its own loads have already read.
+// SC1: agent-scope accesses (L1 bypassed, stores written through).
+typedef uint32_t hs_v4u __attribute__((ext_vector_type(4)));
+constexpr int kRsrcWord3 = 0x00020000;  // raw buffer descriptor, gfx9 family
+
+template <int MODE, bool SC1 = false>
+__device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L) {
+  if (L <= 0) return;
+  const int nt = (int)blockDim.x, t = (int)threadIdx.x;
+  const int nc = nt / kWordsPerLine;  // chains in the region
+  int32_t *base = s + (tid0 / kWordsPerLine) * (long)(L + 1) * kWordsPerLine;
+  const int bytes = nc * (L + 1) * kWordsPerLine * 4;
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(base, 0, bytes, kRsrcWord3);
+  constexpr int kAux = SC1 ? 16 : 0;  // cache policy: sc1
+  for (int hi = L; hi > 0; hi -= 16) {  // source lines [lo, hi)
+    const int lo = hi > 16 ? hi - 16 : 0, nb = hi - lo, items = nc * nb * 4;
+    hs_v4u v[4];
+    int dst[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int q = t + k * nt;
+      dst[k] = -1;
+      if (q < items) {
+        const int part = q & 3, line = lo + (q >> 2) % nb, chain = (q >> 2) / nb;
+        dst[k] = ((chain * (L + 1) + line + 1) * kWordsPerLine + part * 4) * 4;
+        const int src = MODE == 0 ? dst[k] - kWordsPerLine * 4 : dst[k];
+        v[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, src, 0, kAux);
+      }
+    }
+    if (MODE == 0) {  // other lanes store into lines this band still reads
+      wait_mem();
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (dst[k] >= 0)
+        __builtin_amdgcn_raw_buffer_store_b128(MODE == 0 ? v[k] : v[k] + 1u, rs, dst[k], 0, kAux);
+  }
 }
 
 // Semaphore reader body (accessData_semRd, HM:89-100): L loads, summed
@@ -422,7 +471,6 @@ __device__ __forceinline__ bool flag_tree_barrier(const FlagBar &f, int grp,
 //   * the XCD leader writes the XCD's L2 back (release) before arriving at
 //     the device level and invalidates it (acquire) before releasing its XCD.
 // Per episode this is 2 cache operations per XCD instead of 2-3 per WG.
-__device__ __forceinline__ void wait_mem() { __builtin_amdgcn_s_waitcnt(0); }
 
 __device__ __forceinline__ bool poll_ne(const uint32_t *p, uint32_t v,
                                         uint32_t poll_q, Guard &g) {
@@ -459,46 +507,40 @@ struct XcdTree {
 };
 
 // Master thread; every wave of the WG has run wait_mem() + __syncthreads().
-// Arrival climbs group -> XCD -> device; the device releaser writes one
-// generation word per XCD and every WG of that XCD polls its word, so the
-// release is a single hop.
+// Arrival climbs group -> XCD -> device with relaxed RMWs on counters that
+// are never reset (the last arriver of a level is the one whose increment
+// completes a multiple of the level's size); the device releaser writes one
+// release word per XCD and every WG of that XCD polls its word, so the
+// release is a single hop.  `epoch` is the number of barriers of this kind
+// completed once this one is: the value the release word (global) or the
+// group's generation (local_only) must reach, so nothing is read before
+// arriving.  The WGs' data accesses are agent-scope (written through, L1
+// bypassed) and complete before they arrive, so no level needs a cache
+// fence.
 __device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp,
-                                                bool local_only,
+                                                bool local_only, uint32_t epoch,
                                                 uint32_t poll_q, Guard &g) {
+  const uint32_t per = (uint32_t)t.per_cu;
   if (local_only) {  // group-level barrier (local exchange, HM:352-353)
-    if (t.per_cu <= 1) return true;
+    if (per <= 1) return true;
     uint32_t *cnt = t.grp + (long)grp * kBarWords, *gen = cnt + 32;
-    const uint32_t lg = ld_rlx(gen);
-    if (add_rlx(cnt, 1) != (uint32_t)t.per_cu - 1) return poll_ne(gen, lg, poll_q, g);
-    st_rlx(cnt, 0);
-    st_rlx(gen, lg + 1);
+    if ((add_rlx(cnt, 1) + 1) % per != 0) return poll_eq(gen, epoch, poll_q, g);
+    st_rlx(gen, epoch);
     return true;
   }
   const int x = grp % t.nxcd;
   uint32_t *rel = t.rel + (long)x * kBarWords;
-  const uint32_t rv = ld_rlx(rel);  // cannot advance before this WG arrives
-  if (t.per_cu > 1) {
-    uint32_t *cnt = t.grp + (long)grp * kBarWords;
-    if (add_rlx(cnt, 1) != (uint32_t)t.per_cu - 1) return poll_ne(rel, rv, poll_q, g);
-    st_rlx(cnt, 0);
-  }
-  uint32_t *xc = t.xcd + (long)x * kBarWords;
-  if (add_rlx(xc, 1) != (uint32_t)groups_on_xcd(t.nab, t.nxcd, x) - 1)
-    return poll_ne(rel, rv, poll_q, g);
-  st_rlx(xc, 0);
-  // XCD leader: publish whatever the XCD left dirty, then the device level.
-  fence_rel();
-  const int nx = t.nab < t.nxcd ? t.nab : t.nxcd;
-  uint32_t *gc = t.glob, *ggen = t.glob + 32;
-  const uint32_t gg = ld_rlx(ggen);
-  if (add_acqrel(gc, 1) == (uint32_t)nx - 1) {
-    st_rlx(gc, 0);
-    st_rlx(t.episodes, ld_rlx(t.episodes) + 1);
-    st_rel(ggen, gg + 1);
-    for (int y = 0; y < nx; ++y) st_rlx(t.rel + (long)y * kBarWords, rv + 1);
-    return true;
-  }
-  return poll_ne(rel, rv, poll_q, g);
+  if (per > 1 && (add_rlx(t.grp + (long)grp * kBarWords, 1) + 1) % per != 0)
+    return poll_eq(rel, epoch, poll_q, g);
+  const uint32_t ngx = (uint32_t)groups_on_xcd(t.nab, t.nxcd, x);
+  if ((add_rlx(t.xcd + (long)x * kBarWords, 1) + 1) % ngx != 0)
+    return poll_eq(rel, epoch, poll_q, g);
+  const uint32_t nx = (uint32_t)(t.nab < t.nxcd ? t.nab : t.nxcd);
+  if ((add_rlx(t.glob, 1) + 1) % nx != 0) return poll_eq(rel, epoch, poll_q, g);
+  st_rlx(t.episodes, ld_rlx(t.episodes) + 1);
+  st_rlx(t.glob + 32, epoch);
+  for (uint32_t y = 0; y < nx; ++y) st_rlx(t.rel + (long)y * kBarWords, epoch);
+  return true;
 }
 
 struct XcdFlags {
@@ -509,12 +551,20 @@ struct XcdFlags {
   int nab, per_cu, nxcd;
 };
 
-// Whole-WG call.  el = group-level epoch, eg = episode (XCD + device level).
+// Whole-WG call.  el = group-level epoch, eg = episode (device level).
+// Arrival is gathered by flags level by level (group master <- members,
+// XCD leader <- group masters, device master <- XCD leaders).  The release
+// of a global episode is single-hop: the device master writes one release
+// word per XCD (gout[x]) and every WG of that XCD polls its word.  Only the
+// local-exchange barrier releases through the group's out-flags.  Flags
+// and data are agent-scope accesses, so no level needs a cache fence.
 __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int q,
                                                  uint32_t el, uint32_t eg,
                                                  bool local_only, uint32_t poll_q,
                                                  Guard &g, int *s_flag) {
   const int tid = threadIdx.x, nthr = blockDim.x;
+  const int x = grp % f.nxcd;
+  const uint32_t *rel = f.gout + x;
   wait_mem();
   __syncthreads();
   if (f.per_cu > 1) {
@@ -522,7 +572,7 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
     if (q != 0) {
       if (tid == 0) {
         st_rlx(lin + q, el);
-        *s_flag = poll_eq(lout + q, el, poll_q, g);
+        *s_flag = local_only ? poll_eq(lout + q, el, poll_q, g) : poll_eq(rel, eg, poll_q, g);
       }
       __syncthreads();
       return *s_flag != 0;
@@ -530,51 +580,43 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
     bool ok = true;
     for (int m = 1 + tid; m < f.per_cu; m += nthr) ok = ok && poll_eq(lin + m, el, poll_q, g);
     if (!__syncthreads_and(ok)) return false;
-  }
-  if (!local_only) {
-    const int x = grp % f.nxcd, k = grp / f.nxcd;
-    const int ngx = groups_on_xcd(f.nab, f.nxcd, x);
-    if (k != 0) {
-      if (tid == 0) {
-        st_rlx(f.xin + grp, eg);
-        *s_flag = poll_eq(f.xout + grp, eg, poll_q, g);
-      }
-      __syncthreads();
-      if (!*s_flag) return false;
-    } else {
-      bool ok = true;
-      for (int j = 1 + tid; j < ngx; j += nthr) ok = ok && poll_eq(f.xin + x + j * f.nxcd, eg, poll_q, g);
-      if (!__syncthreads_and(ok)) return false;
-      if (tid == 0) {  // XCD leader
-        const int nx = f.nab < f.nxcd ? f.nab : f.nxcd;
-        fence_rel();
-        bool lok = true;
-        if (x != 0) {
-          st_rlx(f.gin + x, eg);
-          lok = spin_until_eq(f.gout + x, eg, poll_q, g);
-        } else {
-          for (int y = 1; y < nx && lok; ++y) lok = poll_eq(f.gin + y, eg, poll_q, g);
-          if (lok) {
-            fence_acq();
-            st_rlx(f.episodes, ld_rlx(f.episodes) + 1);
-            fence_rel();
-            for (int y = 1; y < nx; ++y) st_rlx(f.gout + y, eg);
-          }
-        }
-        if (nx == 1) fence_acq();
-        wait_mem();
-        *s_flag = lok;
-      }
-      __syncthreads();
-      if (!*s_flag) return false;
-      for (int j = 1 + tid; j < ngx; j += nthr) st_rlx(f.xout + x + j * f.nxcd, eg);
+    if (local_only) {
+      for (int m = 1 + tid; m < f.per_cu; m += nthr) st_rlx(lout + m, el);
+      return true;
     }
+  } else if (local_only) {
+    return true;
   }
-  if (f.per_cu > 1) {
-    uint32_t *lout = f.lout + (long)grp * f.per_cu;
+  const int k = grp / f.nxcd;
+  if (k != 0) {  // group master: arrive at the XCD level, wait for the release
+    if (tid == 0) {
+      st_rlx(f.xin + grp, eg);
+      *s_flag = poll_eq(rel, eg, poll_q, g);
+    }
     __syncthreads();
-    for (int m = 1 + tid; m < f.per_cu; m += nthr) st_rlx(lout + m, el);
+    return *s_flag != 0;
   }
+  // XCD leader: gather the XCD's group masters, one flag per lane
+  const int ngx = groups_on_xcd(f.nab, f.nxcd, x);
+  bool ok = true;
+  for (int j = 1 + tid; j < ngx; j += nthr) ok = ok && poll_eq(f.xin + x + j * f.nxcd, eg, poll_q, g);
+  if (!__syncthreads_and(ok)) return false;
+  const int nx = f.nab < f.nxcd ? f.nab : f.nxcd;
+  if (x != 0) {  // XCD leader: arrive at the device level, wait for the release
+    if (tid == 0) {
+      st_rlx(f.gin + x, eg);
+      *s_flag = poll_eq(rel, eg, poll_q, g);
+    }
+    __syncthreads();
+    return *s_flag != 0;
+  }
+  // device master: gather the other XCD leaders, one flag per lane, then
+  // release every XCD
+  ok = true;
+  for (int y = 1 + tid; y < nx; y += nthr) ok = ok && poll_eq(f.gin + y, eg, poll_q, g);
+  if (!__syncthreads_and(ok)) return false;
+  if (tid == 0) st_rlx(f.episodes, ld_rlx(f.episodes) + 1);
+  for (int y = tid; y < nx; y += nthr) st_rlx(f.gout + y, eg);
   return true;
 }
 
