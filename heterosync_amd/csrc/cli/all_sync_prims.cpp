@@ -42,22 +42,36 @@ void usage() {
                "         --strict (agent-scope release/acquire in every WG)\n");
 }
 
-const char *banner(int p) {
-  switch (p) {
-    case 0: return "atomic_tree_barrier";
-    case 1: return "atomic_tree_barrier_localExch";
-    case 2: return "fbs_tree_barrier";
-    case 3: return "fbs_tree_barrier_localExch";
-    case 4: return "spin_lock_mutex";
-    case 5: return "ebo_mutex";
-    case 6: return "sleeping_mutex";
-    case 7: return "fetchadd_mutex";
-    case 20: return "spin_lock_mutex_uniq";
-    case 21: return "ebo_mutex_uniq";
-    case 22: return "sleeping_mutex_uniq";
-    case 23: return "fetchadd_mutex_uniq";
-    default: return nullptr;
+// The line the reference prints before a run (main.hip.cpp:2177-2340),
+// including its semaphore counts: ids 11, 19 and 27 print 2 for their
+// 120-count semaphores there, and so do they here.
+std::string banner(int p, int iters) {
+  static const char *kBarrier[] = {"atomic_tree_barrier", "atomic_tree_barrier_localExch",
+                                   "fbs_tree_barrier", "fbs_tree_barrier_localExch"};
+  static const char *kMutex[] = {"spin_lock_mutex", "ebo_mutex", "sleeping_mutex",
+                                 "fetchadd_mutex"};
+  static const char *kSrb[] = {"atomic_tree_barrier_SRB", "atomic_tree_barrier_localExch_SRB",
+                               "atomic_tree_barrier_NaiveSRB",
+                               "atomic_tree_barrier_localExch_NaiveSRB",
+                               "atomic_tree_barrier_NaiveAllSRB",
+                               "atomic_tree_barrier_localExch_NaiveAllSRB"};
+  static const int kShown[4] = {1, 2, 10, 120};
+  char buf[96];
+  if (p < 4) {
+    std::snprintf(buf, sizeof buf, "%s_%03d", kBarrier[p], iters);
+  } else if (p <= 7 || (p >= 20 && p <= 23)) {
+    std::snprintf(buf, sizeof buf, "%s%s_%03d", kMutex[(p - 4) % 16], p >= 20 ? "_uniq" : "",
+                  iters);
+  } else if (p >= 37) {
+    std::snprintf(buf, sizeof buf, "%s_%03d", kSrb[p - 37], iters);
+  } else {
+    const char *kind = p <= 11 ? "spin_lock_sem" : p <= 15 ? "ebo_sem"
+                     : p <= 19 ? "prior_spin_lock_sem" : p <= 27 ? "spin_lock_sem_uniq"
+                     : p <= 31 ? "ebo_sem_uniq" : "prior_ebo_sem";
+    const int shown = (p == 11 || p == 19 || p == 27) ? 2 : kShown[(p - 8) % 4];
+    std::snprintf(buf, sizeof buf, "%s_%03d_%03d", kind, shown, iters);
   }
+  return buf;
 }
 
 // Passes over each WG-sized region implied by the kernels' region walk.
@@ -65,14 +79,18 @@ bool region_passes(int p, int nw, int iters, int reps, int ncu, std::vector<long
   const long ir = (long)iters * reps;
   const int nab = nw < ncu ? nw : ncu;
   const int per = (nw + nab - 1) / nab;
-  if (p < 4) {
-    k.assign(nw, ir * (((p == 1 || p == 3) && per > 1) ? 2 : 1));
+  long writers = 0;  // semaphores: one writer per group (main.hip.cpp:883)
+  for (int b = 0; b < nw; ++b) writers += (b / nab == (b % nab) % per);
+  if (p < 4 || (p >= 37 && p <= 42)) {  // barriers: every region, twice with local exchange
+    const bool lx = p == 1 || p == 3 || p == 38 || p == 40 || p == 42;
+    k.assign(nw, ir * ((lx && per > 1) ? 2 : 1));
   } else if (p <= 7) {
     k.assign(1, (long)nw * ir);
-  } else if (p <= 15) {
-    long w = 0;
-    for (int b = 0; b < nw; ++b) w += (b / nab == (b % nab) % per);
-    k.assign(per, w * ir);
+  } else if (p <= 19) {  // global semaphores: regions 0..perCU-1
+    k.assign(per, writers * ir);
+  } else if (p >= 32 && p <= 35) {  // global EBO priority kernel over num_wgs regions
+    k.assign(nw, 0);
+    for (int r = 0; r < per && r < nw; ++r) k[r] = writers * ir;
   } else if (p >= 20 && p <= 23) {
     k.assign(nab, 0);
     for (int b = 0; b < nw; ++b) k[b % ncu] += ir;
@@ -164,16 +182,7 @@ int main(int argc, char **argv) {
   std::vector<int32_t> storage(n);
   for (long i = 0; i < n; ++i) storage[i] = (int32_t)i;
 
-  if (const char *b = banner(p)) {
-    std::printf("%s_%03d\n", b, iters);
-  } else {
-    static const int kMax[4] = {1, 2, 10, 120};
-    const bool uniq = p >= 24;
-    const int base = uniq ? 24 : 8;
-    const bool ebo = (p - base) >= 4;
-    std::printf("%s%s_%03d_%03d\n", ebo ? "ebo_sem" : "spin_lock_sem", uniq ? "_uniq" : "",
-                kMax[(p - base) % 4], iters);
-  }
+  std::printf("%s\n", banner(p, iters).c_str());
   std::fflush(stdout);
 
   hs_syncprim_cfg cfg;
@@ -201,7 +210,12 @@ int main(int argc, char **argv) {
   std::fflush(stdout);
 
   std::vector<long> k;
-  region_passes(p, nw, iters, reps, ncu, k);
+  if (!region_passes(p, nw, iters, reps, ncu, k) ||
+      (long)k.size() * uniq_per_wg != n) {  // every storage region has a pass count
+    std::fprintf(stderr, "ERROR: no self-check for %s\n", prim_str);
+    hs_close(ctx);
+    return -1;
+  }
   const long chain = 16L * (L + 1);
   unsigned errors = 0;
   std::printf("Comparing GPU results to golden results:\n");
