@@ -165,6 +165,7 @@ struct UtsArgs {
   // thresholds (geometric)
   const uint32_t *thr;
   int nthr;
+  float inv_log2q;  // 1 / log2(1 - 1/(1+b0)): the count's float estimate
   // reference tree
   const uint32_t *value;
   const int32_t *child_a, *child_b;
@@ -270,6 +271,7 @@ template <int TREE, bool SERVE>
 __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
   constexpr int W = Rec<TREE>::W;
   __shared__ uint32_t s_pst[W * kWave];   // parent records of the batch
+  __shared__ uint32_t s_top[W * kWave];   // newest children, word-major, not in the deque
   __shared__ uint32_t s_off[kWave + 1];   // exclusive scan of child counts
   __shared__ int s_hold[kMaxSlots];       // nodes of each slot held by this WG
   __shared__ int s_cred[kMaxSlots];       // credits of each slot held by this WG
@@ -374,6 +376,22 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
     __syncthreads();
   };
 
+  // The newest lds_n (<= 64) records of the deque -- the last batch's last
+  // children -- are held only in s_top: a depth-first pop takes exactly
+  // them, so they never make the round trip through HBM.  flush_top() stores
+  // them to their deque slots [top - lds_n, top) when the deque itself must
+  // be complete (a breadth-first pop, a delivery appended at the top).
+  int lds_n = 0;
+  auto flush_top = [&]() {
+    if (lds_n == 0) return;
+    if (lane < lds_n) {
+      const long at = (top - lds_n + lane) & dq_mask;
+#pragma unroll
+      for (int w = 0; w < W; ++w) deq[dq_idx<W>(at, w)] = s_top[w * kWave + lane];
+    }
+    lds_n = 0;
+  };
+
   // Takes the k records a donor (or the host, for a root) left in this WG's
   // inbox; their credits come with them.  Whole wave.
   auto absorb = [&](int k) -> bool {
@@ -384,6 +402,7 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
       }
       return false;
     }
+    flush_top();  // the delivery lands above the current top
     fence_acq();  // the donor released the records with the count
     if (SERVE)
       for (int s2 = lane; s2 < kMaxSlots; s2 += kWave) s_gmx[s2] = ld_rlx(a.slot_gen_mx + s2);
@@ -483,12 +502,18 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
       // are cut from.
       const int n = held < kWave ? (int)held : kWave;
       const bool bfs = held < kGrow;
+      if (bfs) flush_top();  // pops from the bottom; this batch refills s_top
       uint32_t nc = 0, meta = 0, slot = 0, dep = 0, lval = 0;
       if (lane < n) {
         const long at = (bfs ? bot + lane : top - 1 - lane) & dq_mask;
         uint32_t r[W];
+        if (lane < lds_n) {  // depth-first: the i-th newest is s_top slot lds_n-1-i
 #pragma unroll
-        for (int w = 0; w < W; ++w) r[w] = deq[dq_idx<W>(at, w)];
+          for (int w = 0; w < W; ++w) r[w] = s_top[w * kWave + (lds_n - 1 - lane)];
+        } else {
+#pragma unroll
+          for (int w = 0; w < W; ++w) r[w] = deq[dq_idx<W>(at, w)];
+        }
 #pragma unroll
         for (int w = 0; w < W; ++w) s_pst[lane * W + w] = r[w];
         meta = r[W - 1];
@@ -497,12 +522,15 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
           const uint32_t depth = meta_depth(meta);
           const uint32_t h = r[4] & 0x7fffffffu;
           if (depth < s_gmx[slot] || depth == 0) {
-            int lo = 0, hi = a.nthr;  // count thresholds <= h
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (s_thr[mid] <= h) lo = mid + 1; else hi = mid;
-            }
-            nc = lo;
+            // #thresholds <= h: start from the closed form
+            // floor(log(1 - h/2^31) / log(1 - p)) in float, then step to
+            // the exact count on the integer table (usually 0-1 steps)
+            const float u = (float)h * (1.0f / 2147483648.0f);
+            const float est = __log2f(1.0f - u) * a.inv_log2q;
+            int k = est < 0.0f ? 0 : (est > (float)a.nthr ? a.nthr : (int)est);
+            while (k > 0 && s_thr[k - 1] > h) --k;
+            while (k < a.nthr && s_thr[k] <= h) ++k;
+            nc = k;
           }
           dep = depth;
         } else {
@@ -511,7 +539,13 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
           if (ca == 0) lval = a.value[r[0]];
         }
       }
-      // per-tree accounting: one lane when the batch is all one tree
+      // child offsets (exclusive scan) and the batch total
+      uint32_t total;
+      const uint32_t off = wave_excl_scan(lane < n ? nc : 0, &total);
+      s_off[lane] = off;
+      if (lane == 63) s_off[64] = off + (lane < n ? nc : 0);
+      // per-tree accounting: one lane when the batch is all one tree (the
+      // held count changes by total - n: children in, parents out)
       {
         const uint32_t slot0 = __shfl(slot, 0, 64);
         const bool active = lane < n;
@@ -519,18 +553,19 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
           const unsigned long long leafmask = __ballot(active && nc == 0);
           uint32_t dmax = active ? dep : 0;
           unsigned long long lsum = active ? lval : 0;
-          int dnc = active ? (int)nc - 1 : 0;
 #pragma unroll
           for (int o = 32; o > 0; o >>= 1) {
-            const uint32_t od = __shfl_xor(dmax, o, 64);
-            dmax = od > dmax ? od : dmax;
-            lsum += __shfl_xor(lsum, o, 64);
-            dnc += __shfl_xor(dnc, o, 64);
+            if (TREE == 1) {
+              const uint32_t od = __shfl_xor(dmax, o, 64);
+              dmax = od > dmax ? od : dmax;
+            } else {
+              lsum += __shfl_xor(lsum, o, 64);
+            }
           }
           if (lane == 0) {
             s_nodes[slot0] += (unsigned long long)n;
             s_leaves[slot0] += (unsigned long long)__popcll(leafmask);
-            s_hold[slot0] += dnc;
+            s_hold[slot0] += (int)total - n;
             if (TREE == 1 && dmax > s_maxd[slot0]) s_maxd[slot0] = dmax;
             if (TREE == 0) s_lsum[slot0] += lsum;
           }
@@ -543,10 +578,7 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
         }
       }
       if (bfs) bot += n; else top -= n;
-      uint32_t total;
-      const uint32_t off = wave_excl_scan(lane < n ? nc : 0, &total);
-      s_off[lane] = off;
-      if (lane == 63) s_off[64] = off + (lane < n ? nc : 0);
+      lds_n = 0;  // a depth-first pop of n >= lds_n took them all
       __syncthreads();
       if (top + total - bot > dq_cap) {  // would overflow the ring
         if (lane == 0) {
@@ -557,6 +589,9 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
       }
       phase(st_tpop);
       // ---- generate the children, 64 per round ---------------------------
+      // the newest min(total, 64) children stay in s_top (slot c - keep0)
+      const uint32_t keep = total < (uint32_t)kWave ? total : (uint32_t)kWave;
+      const uint32_t keep0 = total - keep;
       for (uint32_t base = 0; base < total; base += kWave) {
         ++st_rounds;
         const uint32_t c = base + lane;
@@ -573,17 +608,31 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
             uint32_t ch[5];
             const uint32_t par[5] = {pr[0], pr[1], pr[2], pr[3], pr[4]};
             uts_spawn(par, i, ch);
+            if (c >= keep0) {
 #pragma unroll
-            for (int w = 0; w < 5; ++w) deq[dq_idx<W>(at, w)] = ch[w];
-            deq[dq_idx<W>(at, 5)] = pr[5] + 1;  // depth + 1, same slot
+              for (int w = 0; w < 5; ++w) s_top[w * kWave + (c - keep0)] = ch[w];
+              s_top[5 * kWave + (c - keep0)] = pr[5] + 1;  // depth + 1, same slot
+            } else {
+#pragma unroll
+              for (int w = 0; w < 5; ++w) deq[dq_idx<W>(at, w)] = ch[w];
+              deq[dq_idx<W>(at, 5)] = pr[5] + 1;
+            }
           } else {
             const uint32_t node = pr[0];
-            deq[dq_idx<W>(at, 0)] = (c - s_off[lo]) == 0 ? (uint32_t)a.child_a[node] : (uint32_t)a.child_b[node];
-            deq[dq_idx<W>(at, 1)] = pr[1];
+            const uint32_t kid =
+                (c - s_off[lo]) == 0 ? (uint32_t)a.child_a[node] : (uint32_t)a.child_b[node];
+            if (c >= keep0) {
+              s_top[c - keep0] = kid;
+              s_top[kWave + (c - keep0)] = pr[1];
+            } else {
+              deq[dq_idx<W>(at, 0)] = kid;
+              deq[dq_idx<W>(at, 1)] = pr[1];
+            }
           }
         }
       }
       top += total;
+      lds_n = (int)keep;
       ++batches;
       st_nodes += n;
       __syncthreads();
@@ -802,6 +851,7 @@ struct Engine {
     a.admitted = a.live + 32;
     if (tree == 1) {
       std::vector<uint32_t> thr;
+      a.inv_log2q = (float)(1.0 / std::log2(1.0 - 1.0 / (1.0 + b0)));
       a.nthr = geo_thresholds(b0, thr);
       if (a.nthr <= 0 || a.nthr > kMaxThr) return HS_EINVAL;
       HS_HIP(hipMemcpy(d_thr, thr.data(), thr.size() * 4, hipMemcpyHostToDevice));
