@@ -72,8 +72,13 @@ __device__ __forceinline__ void fence_rel() {
 }
 
 // s_sleep takes an immediate; sleep `n` quanta (~64 clocks each) by binary
-// decomposition (the role of sleepFunc, HL:22-69).
+// decomposition (the role of sleepFunc, HL:22-69).  s_sleep is a scalar
+// instruction and ignores the exec mask, so the count is taken from the
+// first active lane: a count the compiler keeps in a vector register (one
+// derived from a loaded value) would otherwise run every s_sleep of the
+// decomposition.  Callers pass a wave-uniform count.
 __device__ __forceinline__ void sleep_quanta(uint32_t n) {
+  n = __builtin_amdgcn_readfirstlane(n);
   while (n >= 127) {
     __builtin_amdgcn_s_sleep(127);
     n -= 127;

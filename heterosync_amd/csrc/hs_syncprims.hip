@@ -4,9 +4,9 @@
 // region of interest (main.hip.cpp:2354-2511).
 //
 // Kernels (one template instance per primitive family):
-//   k_mutex<KIND, UNIQ, MODE>   spin/EBO/sleep/FA mutex, global or per-CU
+//   k_mutex<KIND, UNIQ, MODE, AGENT>  spin/EBO/sleep/FA mutex, global or per-CU
 //                               (main.hip.cpp:645-857)
-//   k_sem<EBO, UNIQ, MODE>      spin/EBO reader-writer semaphores
+//   k_sem<EBO, UNIQ, MODE, PRIO, AGENT>  spin/EBO reader-writer semaphores
 //                               (main.hip.cpp:861-930, 1005-1073,
 //                                1147-1214, 1285-1352)
 //   k_barrier<FLAG, LX, MODE>   atomic / lock-free tree barrier, with and
@@ -42,7 +42,11 @@ struct SPArgs {
   uint32_t *flat;     // sense-reversing family: main block, local block
 };
 
-template <int KIND, bool UNIQ, int MODE>
+// AGENT: critical-section data through agent-scope accesses, drained by
+// every wave before the WG barrier that precedes the release, so the lock
+// and semaphore words need no release/acquire fences (the strict form keeps
+// them and plain data accesses).
+template <int KIND, bool UNIQ, int MODE, bool AGENT>
 __global__ void __launch_bounds__(1024) k_mutex(SPArgs a) {
   __shared__ int s_ok;
   Guard g;
@@ -57,19 +61,20 @@ __global__ void __launch_bounds__(1024) k_mutex(SPArgs a) {
   __syncthreads();
   if (!s_ok) return;
   for (int i = 0; i < a.iters; ++i) {
-    if (is_master()) s_ok = mutex_lock<KIND>(lk, ring, a.ring_stride, ticket, a.bo, g);
+    if (is_master()) s_ok = mutex_lock<KIND, AGENT>(lk, ring, a.ring_stride, ticket, a.bo, g);
     __syncthreads();
     if (!s_ok) return;
-    wg_move<MODE>(a.storage, tid0, a.L);
+    wg_move<MODE, AGENT>(a.storage, tid0, a.L);
+    if (AGENT) wait_mem();
     __syncthreads();
     if (is_master()) {
-      st_rlx(lk + kLkCs, ld_rlx(lk + kLkCs) + 1);  // plain, under the lock
-      mutex_unlock<KIND>(lk, ring, a.ring_stride, ticket);
+      mutex_unlock<KIND, AGENT>(lk, ring, a.ring_stride, ticket);
+      add_rlx(lk + kLkCs, 1);  // entry count, off the hand-off path (no return)
     }
   }
 }
 
-template <bool EBO, bool UNIQ, int MODE, bool PRIO = false>
+template <bool EBO, bool UNIQ, int MODE, bool PRIO, bool AGENT>
 __global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
   __shared__ int s_ok;
   __shared__ int32_t sink[1024];
@@ -84,30 +89,28 @@ __global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
   __syncthreads();
   if (!s_ok) return;
   for (int i = 0; i < a.iters; ++i) {
-    if (is_master()) s_ok = sem_wait<EBO, PRIO>(lk, writer, a.bo, g);
+    if (is_master()) s_ok = sem_wait<EBO, PRIO, AGENT>(lk, writer, a.bo, g);
     __syncthreads();
     if (!s_ok) return;
     if (writer) {
       if (UNIQ) {  // main.hip.cpp:1040-1058: own region, then +numCU WGs
         int cur = b;
         for (int j = 0; j < per; ++j) {
-          wg_move<MODE>(a.storage, (long)cur * blockDim.x, a.L);
+          wg_move<MODE, AGENT>(a.storage, (long)cur * blockDim.x, a.L);
           cur = (cur + nab) % grid;
         }
       } else {  // main.hip.cpp:903-915: regions 0..perCU-1
         for (int j = 0; j < per; ++j)
-          wg_move<MODE>(a.storage, (long)j * blockDim.x, a.L);
+          wg_move<MODE, AGENT>(a.storage, (long)j * blockDim.x, a.L);
       }
     } else {
-      wg_read(a.storage, (long)(UNIQ ? b : pw) * blockDim.x, a.L, sink);
+      wg_read<AGENT>(a.storage, (long)(UNIQ ? b : pw) * blockDim.x, a.L, sink);
     }
+    if (AGENT) wait_mem();
     __syncthreads();
     if (is_master()) {
-      if (writer)
-        st_rlx(lk + kLkCs, ld_rlx(lk + kLkCs) + 1);  // exclusive
-      else
-        add_rlx(lk + kLkRd, 1);
-      s_ok = sem_post<PRIO>(lk, writer, a.bo, g);
+      s_ok = sem_post<PRIO, AGENT>(lk, writer, a.bo, g);
+      add_rlx(lk + (writer ? kLkCs : kLkRd), 1);  // entry counts, off the hand-off path
     }
     __syncthreads();
     if (!s_ok) return;
@@ -284,14 +287,14 @@ KernelFn pick_kernel(int p) {
     case 1: return k_barrier<false, true, MODE, XCD>;
     case 2: return k_barrier<true, false, MODE, XCD>;
     case 3: return k_barrier<true, true, MODE, XCD>;
-    case 4: return k_mutex<kSpin, false, MODE>;
-    case 5: return k_mutex<kEBO, false, MODE>;
-    case 6: return k_mutex<kSleep, false, MODE>;
-    case 7: return k_mutex<kFA, false, MODE>;
-    case 20: return k_mutex<kSpin, true, MODE>;
-    case 21: return k_mutex<kEBO, true, MODE>;
-    case 22: return k_mutex<kSleep, true, MODE>;
-    case 23: return k_mutex<kFA, true, MODE>;
+    case 4: return k_mutex<kSpin, false, MODE, XCD>;
+    case 5: return k_mutex<kEBO, false, MODE, XCD>;
+    case 6: return k_mutex<kSleep, false, MODE, XCD>;
+    case 7: return k_mutex<kFA, false, MODE, XCD>;
+    case 20: return k_mutex<kSpin, true, MODE, XCD>;
+    case 21: return k_mutex<kEBO, true, MODE, XCD>;
+    case 22: return k_mutex<kSleep, true, MODE, XCD>;
+    case 23: return k_mutex<kFA, true, MODE, XCD>;
     case 37: return k_srb<0, MODE, XCD>;
     case 38: return k_srb<1, MODE, XCD>;
     case 39: return k_srb<2, MODE, XCD>;
@@ -300,12 +303,12 @@ KernelFn pick_kernel(int p) {
     case 42: return k_srb<5, MODE, XCD>;
     default: break;
   }
-  if (p >= 8 && p <= 11) return k_sem<false, false, MODE>;
-  if (p >= 12 && p <= 15) return k_sem<true, false, MODE>;
-  if (p >= 24 && p <= 27) return k_sem<false, true, MODE>;
-  if (p >= 28 && p <= 31) return k_sem<true, true, MODE>;
-  if (p >= 16 && p <= 19) return k_sem<false, false, MODE, true>;
-  if (p >= 32 && p <= 35) return k_sem<true, false, MODE, true>;
+  if (p >= 8 && p <= 11) return k_sem<false, false, MODE, false, XCD>;
+  if (p >= 12 && p <= 15) return k_sem<true, false, MODE, false, XCD>;
+  if (p >= 24 && p <= 27) return k_sem<false, true, MODE, false, XCD>;
+  if (p >= 28 && p <= 31) return k_sem<true, true, MODE, false, XCD>;
+  if (p >= 16 && p <= 19) return k_sem<false, false, MODE, true, XCD>;
+  if (p >= 32 && p <= 35) return k_sem<true, false, MODE, true, XCD>;
   return nullptr;
 }
 
@@ -337,6 +340,10 @@ extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
                                          std::min(cfg->num_wgs, ncu)
                                    : 1;
   const bool xcd_form = (p >= 0 && p < 4) || ((p == 37 || p == 38) && per > 4);
+  // mutexes and semaphores: the fence-free (agent-scope data) form needs no
+  // placement, so it runs directly
+  if (cfg->sync_scope == 0 && (is_mutex(p) || is_sem(p)))
+    return run_once(c, cfg, storage, n, res, true, nullptr);
   const bool xcd = cfg->sync_scope == 0 && xcd_form && num_xcds(c) > 1;
   if (!xcd) return run_once(c, cfg, storage, n, res, false, nullptr);
   std::vector<int32_t> init(storage, storage + (n > 0 ? n : 0));

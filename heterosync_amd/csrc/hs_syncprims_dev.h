@@ -159,13 +159,14 @@ __device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L) {
 
 // Semaphore reader body (accessData_semRd, HM:89-100): L loads, summed
 // into LDS so they are not dead.
+template <bool SC1 = false>
 __device__ __forceinline__ void wg_read(const int32_t *s, long tid0, int L,
                                         volatile int32_t *sink) {
   const long tid = tid0 + threadIdx.x;
   const int32_t *col = s + (tid / kWordsPerLine) * (L + 1) * kWordsPerLine +
                        threadIdx.x % kWordsPerLine;
   int32_t acc = 0;
-  for (int n = L - 1; n >= 0; --n) acc += col[n * kWordsPerLine];
+  for (int n = L - 1; n >= 0; --n) acc += ld_data<SC1>(col + n * kWordsPerLine);
   sink[threadIdx.x] += acc;
 }
 
@@ -196,14 +197,18 @@ __device__ __forceinline__ bool spin_until_ne(const uint32_t *p, uint32_t v,
 //   EBO   : same with capped exponential s_sleep backoff (HME:14-66)
 //   FA    : ticket lock, ticket = word 0, turn = word 32 (HMF:14-75)
 //   sleep : array queue lock over a ring of slots (HMZ:22-123)
-template <int KIND>
+// AGENT: the critical section's data accesses are agent-scope (L1 bypassed,
+// stores written through) and every wave drains them before the WG releases
+// the lock, so acquiring and releasing need no cache fence.
+template <int KIND, bool AGENT = false>
 __device__ __forceinline__ bool mutex_lock(uint32_t *lk, uint32_t *ring,
                                            int ring_stride, uint32_t &ticket,
                                            const Backoff &bo, Guard &g) {
   if (KIND == kSpin || KIND == kEBO) {
     uint32_t b = bo.min_q;
     for (;;) {
-      if (ld_rlx(lk + kLkHead) == 0 && cas_acq(lk + kLkHead, 0, 1) == 0)
+      if (ld_rlx(lk + kLkHead) == 0 &&
+          (AGENT ? cas_rlx(lk + kLkHead, 0, 1) : cas_acq(lk + kLkHead, 0, 1)) == 0)
         return true;
       if (!g.ok()) return false;
       if (KIND == kEBO) {
@@ -224,33 +229,45 @@ __device__ __forceinline__ bool mutex_lock(uint32_t *lk, uint32_t *ring,
       const uint32_t ahead = ticket - turn;
       sleep_quanta(bo.poll_q * (ahead < 8 ? ahead : 8));
     }
-    fence_acq();
+    if (!AGENT) fence_acq();
     return true;
   } else {  // kSleep
     ticket = add_rlx(lk + kLkTail, 1);
     uint32_t *slot = ring + (long)ticket * ring_stride;
     uint32_t b = bo.min_q;
-    while (ld_rlx(slot) != 1u) {
+    for (;;) {
+      const uint32_t v = ld_rlx(slot);
+      const uint32_t done = ld_rlx(lk + kLkCs);  // entries completed (read alongside)
+      if (v == 1u) break;
       if (!g.ok()) return false;
-      sleep_quanta(b);
+      // the reference's exponential schedule (HMS:52-56), capped by this
+      // waiter's place in the queue: it polls a slot of its own, so a sleep
+      // longer than the hand-offs ahead of it only delays its own turn
+      const uint32_t ahead = ticket - done;
+      const uint32_t cap = bo.poll_q * (ahead < 8 ? ahead : 8);
+      sleep_quanta(b < cap ? b : cap);
       b = ((b << 1) + 1) & bo.max_q;
       if (b < bo.min_q) b = bo.min_q;
     }
-    fence_acq();
+    if (!AGENT) fence_acq();
     return true;
   }
 }
 
-template <int KIND>
+// Master thread, after every wave's critical-section accesses (AGENT: and
+// their completion) and a WG barrier.
+template <int KIND, bool AGENT = false>
 __device__ __forceinline__ void mutex_unlock(uint32_t *lk, uint32_t *ring,
                                              int ring_stride, uint32_t ticket) {
+  if (AGENT) wait_mem();  // this wave's own stores (the CS counter) first
   if (KIND == kSpin || KIND == kEBO) {
-    st_rel(lk + kLkHead, 0);
+    if (AGENT) st_rlx(lk + kLkHead, 0); else st_rel(lk + kLkHead, 0);
   } else if (KIND == kFA) {
-    st_rel(lk + kLkTail, ticket + 1);
+    if (AGENT) st_rlx(lk + kLkTail, ticket + 1); else st_rel(lk + kLkTail, ticket + 1);
   } else {
     st_rlx(ring + (long)ticket * ring_stride, 0xffffffffu);
-    st_rel(ring + (long)(ticket + 1) * ring_stride, 1u);
+    if (AGENT) st_rlx(ring + (long)(ticket + 1) * ring_stride, 1u);
+    else st_rel(ring + (long)(ticket + 1) * ring_stride, 1u);
   }
 }
 
@@ -268,7 +285,8 @@ constexpr uint32_t kSemWaitBit = 1u << 16;
 
 __device__ __forceinline__ uint32_t sem_init_word(uint32_t max) { return max | kSemWaitBit; }
 
-// Returns 1 = entered, 0 = not yet, -1 = aborted.
+// Returns 1 = entered, 0 = not yet, -1 = aborted.  AGENT as for mutexes.
+template <bool AGENT = false>
 __device__ __forceinline__ int sem_try_wait(uint32_t *lk, bool writer, Guard &g) {
   const uint32_t mx = ld_rlx(lk + kLkMax);
   uint32_t s = ld_rlx(lk + kLkHead);
@@ -287,7 +305,8 @@ __device__ __forceinline__ int sem_try_wait(uint32_t *lk, bool writer, Guard &g)
       if (!enter) return g.ok() ? 0 : -1;
       want = s - 1;
     }
-    const uint32_t old = enter ? cas_acq(lk + kLkHead, s, want) : cas_rlx(lk + kLkHead, s, want);
+    const uint32_t old =
+        (enter && !AGENT) ? cas_acq(lk + kLkHead, s, want) : cas_rlx(lk + kLkHead, s, want);
     if (old == s) return enter ? 1 : (g.ok() ? 0 : -1);
     s = old;  // raced with another update: re-evaluate
     if (!g.ok()) return -1;
@@ -308,13 +327,13 @@ __device__ __forceinline__ bool sem_yield_to_posts(const uint32_t *lk, const Bac
   return true;
 }
 
-template <bool EBO, bool PRIO = false>
+template <bool EBO, bool PRIO = false, bool AGENT = false>
 __device__ __forceinline__ bool sem_wait(uint32_t *lk, bool writer,
                                          const Backoff &bo, Guard &g) {
   uint32_t b = bo.min_q;
   for (;;) {
     if (PRIO && !sem_yield_to_posts(lk, bo, g)) return false;
-    const int r = sem_try_wait(lk, writer, g);
+    const int r = sem_try_wait<AGENT>(lk, writer, g);
     if (r > 0) return true;
     if (r < 0) return false;
     if (EBO) {
@@ -333,11 +352,16 @@ __device__ __forceinline__ bool sem_wait(uint32_t *lk, bool writer,
 // Post: one release RMW (also publishes a writer's CS data).  The priority
 // form brackets it with the in-flight count that entering WGs yield to
 // (HSS:591-595, 639-640).
-template <bool PRIO = false>
+template <bool PRIO = false, bool AGENT = false>
 __device__ __forceinline__ bool sem_post(uint32_t *lk, bool writer,
                                          const Backoff &, Guard &) {
   if (PRIO) add_rlx(lk + kLkPrio, 1u);
-  add_rel(lk + kLkHead, writer ? ld_rlx(lk + kLkMax) : 1u);
+  if (AGENT) {
+    wait_mem();  // this wave's own stores (the CS counters) first
+    add_rlx(lk + kLkHead, writer ? ld_rlx(lk + kLkMax) : 1u);
+  } else {
+    add_rel(lk + kLkHead, writer ? ld_rlx(lk + kLkMax) : 1u);
+  }
   if (PRIO) add_rlx(lk + kLkPrio, 0xffffffffu);
   return true;
 }

@@ -36,45 +36,54 @@ def check(dev, ohelp, prim, L, nw, iters, reps=10, wg=64, mode=0, literal=False,
     return r
 
 
+# sync_scope 0: fence-free hand-off over agent-scope data; 1: strict
+# release/acquire with plain data accesses
 @pytest.mark.parametrize("prim", MUTEX)
 @pytest.mark.parametrize("mode", [0, 1])
-def test_mutex(dev, ohelp, prim, mode):
-    check(dev, ohelp, prim, 10, 32, 20, mode=mode)
-    check(dev, ohelp, prim, 3, 4, 5, mode=mode, literal=True)
+@pytest.mark.parametrize("scope", [0, 1])
+def test_mutex(dev, ohelp, prim, mode, scope):
+    r = check(dev, ohelp, prim, 10, 32, 20, mode=mode, sync_scope=scope)
+    assert r.sync_scope_used == scope
+    check(dev, ohelp, prim, 3, 4, 5, mode=mode, literal=True, sync_scope=scope)
 
 
 @pytest.mark.parametrize("prim", UMUTEX)
-def test_uniq_mutex(dev, ohelp, prim):
+@pytest.mark.parametrize("scope", [0, 1])
+def test_uniq_mutex(dev, ohelp, prim, scope):
     ncu = dev.info.num_cu
-    check(dev, ohelp, prim, 4, 2 * ncu, 10, mode=1)
-    check(dev, ohelp, prim, 4, ncu + 3, 5, mode=1)   # uneven WGs per lock
-    check(dev, ohelp, prim, 2, 8, 5, mode=0, literal=True)
+    check(dev, ohelp, prim, 4, 2 * ncu, 10, mode=1, sync_scope=scope)
+    check(dev, ohelp, prim, 4, ncu + 3, 5, mode=1, sync_scope=scope)   # uneven WGs per lock
+    check(dev, ohelp, prim, 2, 8, 5, mode=0, literal=True, sync_scope=scope)
 
 
 @pytest.mark.parametrize("prim", SEM)
-def test_semaphore(dev, ohelp, prim):
+@pytest.mark.parametrize("scope", [0, 1])
+def test_semaphore(dev, ohelp, prim, scope):
     ncu = dev.info.num_cu
-    check(dev, ohelp, prim, 4, 16, 10, mode=1)
-    check(dev, ohelp, prim, 4, 2 * ncu, 2, reps=2, mode=1)  # 1 writer + readers per CU
+    r = check(dev, ohelp, prim, 4, 16, 10, mode=1, sync_scope=scope)
+    assert r.sync_scope_used == scope
+    check(dev, ohelp, prim, 4, 2 * ncu, 2, reps=2, mode=1, sync_scope=scope)  # 1 writer + readers per CU
 
 
 @pytest.mark.parametrize("prim", PRIO)
 @pytest.mark.parametrize("mode", [0, 1])
-def test_priority_semaphore(dev, ohelp, prim, mode):
+@pytest.mark.parametrize("scope", [0, 1])
+def test_priority_semaphore(dev, ohelp, prim, mode, scope):
     """Entries yield to posts in flight; 32-35 run the global kernel over
     num_wgs regions (the reference's dispatch), so only regions < perCU move."""
     ncu = dev.info.num_cu
-    check(dev, ohelp, prim, 4, 16, 10, mode=mode)
-    check(dev, ohelp, prim, 4, 2 * ncu, 2, reps=2, mode=mode)
-    r = check(dev, ohelp, prim, 2, 4 * ncu, 3, reps=2, mode=mode)
-    assert r.state_ok == 1
+    check(dev, ohelp, prim, 4, 16, 10, mode=mode, sync_scope=scope)
+    check(dev, ohelp, prim, 4, 2 * ncu, 2, reps=2, mode=mode, sync_scope=scope)
+    r = check(dev, ohelp, prim, 2, 4 * ncu, 3, reps=2, mode=mode, sync_scope=scope)
+    assert r.state_ok == 1 and r.sync_scope_used == scope
 
 
 @pytest.mark.parametrize("prim", USEM)
-def test_uniq_semaphore(dev, ohelp, prim):
+@pytest.mark.parametrize("scope", [0, 1])
+def test_uniq_semaphore(dev, ohelp, prim, scope):
     ncu = dev.info.num_cu
-    check(dev, ohelp, prim, 4, 2 * ncu, 5, mode=1)
-    check(dev, ohelp, prim, 2, 4 * ncu, 3, reps=3, mode=0)
+    check(dev, ohelp, prim, 4, 2 * ncu, 5, mode=1, sync_scope=scope)
+    check(dev, ohelp, prim, 2, 4 * ncu, 3, reps=3, mode=0, sync_scope=scope)
 
 
 @pytest.mark.parametrize("prim", BARRIER)
@@ -121,8 +130,9 @@ def test_srb_full_config(dev, ohelp, prim):
 def test_fa_mutex_c1(dev, ohelp):
     """faMutex 32 WG x 64, numLdSt=10, 100 iterations: 32000 acquires."""
     for mode in (0, 1):
-        r = check(dev, ohelp, 7, 10, 32, 100, mode=mode)
-        assert r.lock_words[0] == r.lock_words[4] == 32000  # ticket == turn
+        for scope in (0, 1):
+            r = check(dev, ohelp, 7, 10, 32, 100, mode=mode, sync_scope=scope)
+            assert r.lock_words[0] == r.lock_words[4] == 32000  # ticket == turn
 
 
 def test_error_paths(dev):
