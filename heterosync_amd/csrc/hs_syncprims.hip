@@ -193,8 +193,8 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
 // flat barrier (VAR 3, 5).  VAR 3 re-visits its own region, because its
 // exchange stride is the grid (main.hip.cpp:472-474, 1568); VAR 1 and 5
 // step by the group count (main.hip.cpp:411-413, 513-515).  XCD selects
-// the XCD-aware tree and write-through data accesses; the host uses it
-// only for the tree form, flat barriers always release/acquire per arrival.
+// agent-scope data accesses with fence-free arrivals: the XCD-aware tree
+// (which checks the WGs' placement) or the flat barrier (which does not).
 template <int VAR, int MODE, bool XCD>
 __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
   __shared__ int s_ok;
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
   int cur = b;
   if (is_master()) {
     s_ok = !g.aborted();
-    if (XCD) {
+    if (XCD && tree) {  // only the XCD tree depends on where WGs run
       const uint32_t me = xcc_id();
       const uint32_t old = cas_rlx(a.xcd_map + b % a.nxcd, 0xffffffffu, me);
       if (old != 0xffffffffu && old != me) {
@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
       uint32_t *blk = a.flat + (local ? kBarWords : 0);
       uint32_t *ep = local ? nullptr : a.tb.episodes;
       if (all ? wave_lead : is_master())
-        ok = flat_arrive(blk, all ? 64u : 1u, parts, ep, a.bo.poll_q, g);
+        ok = flat_arrive<XCD>(blk, all ? 64u : 1u, parts, ep, a.bo.poll_q, g);
     }
     return __syncthreads_and(ok) != 0;
   };
@@ -340,9 +340,10 @@ extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
                                          std::min(cfg->num_wgs, ncu)
                                    : 1;
   const bool xcd_form = (p >= 0 && p < 4) || ((p == 37 || p == 38) && per > 4);
-  // mutexes and semaphores: the fence-free (agent-scope data) form needs no
-  // placement, so it runs directly
-  if (cfg->sync_scope == 0 && (is_mutex(p) || is_sem(p)))
+  // mutexes, semaphores and the flat sense-reversing barriers: the fence-free
+  // (agent-scope data) form needs no placement, so it runs directly
+  const bool srb_flat = p >= 37 && p <= 42 && !xcd_form;
+  if (cfg->sync_scope == 0 && (is_mutex(p) || is_sem(p) || srb_flat))
     return run_once(c, cfg, storage, n, res, true, nullptr);
   const bool xcd = cfg->sync_scope == 0 && xcd_form && num_xcds(c) > 1;
   if (!xcd) return run_once(c, cfg, storage, n, res, false, nullptr);
@@ -571,9 +572,13 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
       };
       ok = ok && flat_ok(0);
       if ((p == 40 || p == 42) && per > 1) ok = ok && flat_ok(1);
-      if (p == 38 && per > 1)  // per-group local barrier (release/acquire form)
+      if (p == 38 && per > 1) {  // per-group local barrier
+        // strict: counter reset each episode; fence-free: never reset
+        const size_t g0 = xcd_mode ? kBarWords * (1 + nxcd) : kBarWords;
+        const uint32_t cnt_want = xcd_mode ? te * (uint32_t)per : 0u;
         for (int gi = 0; gi < nab; ++gi)
-          ok = ok && bar[kBarWords * (1 + gi)] == 0 && bar[kBarWords * (1 + gi) + 32] == te;
+          ok = ok && bar[g0 + kBarWords * gi] == cnt_want && bar[g0 + kBarWords * gi + 32] == te;
+      }
       const uint32_t w[8] = {flat[0], flat[1], flat[32], flat[kBarWords + 32], ep, 0, 0, 0};
       std::memcpy(res->lock_words, w, sizeof(w));
     } else if (p <= 1 || srb) {

@@ -655,19 +655,23 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
 // that a fast WG of the next episode could overtake.  The last arriver
 // publishes generation e+1 with a release store; waiters acquire it.
 //   blk[0..1] arrival count (64-bit), blk[32] generation.
+// AGENT: the participants' data accesses are agent-scope and drained before
+// they arrive, so the arrival and the generation need no fences.
+template <bool AGENT = false>
 __device__ __forceinline__ bool flat_arrive(uint32_t *blk, uint32_t weight, uint64_t parts,
                                             uint32_t *episodes, uint32_t poll_q, Guard &g) {
   uint64_t *cnt = reinterpret_cast<uint64_t *>(blk);
   uint32_t *gen = blk + 32;
-  const uint64_t old =
-      __hip_atomic_fetch_add(cnt, (uint64_t)weight, __ATOMIC_ACQ_REL, HS_AGENT);
+  const uint64_t old = AGENT
+      ? __hip_atomic_fetch_add(cnt, (uint64_t)weight, __ATOMIC_RELAXED, HS_AGENT)
+      : __hip_atomic_fetch_add(cnt, (uint64_t)weight, __ATOMIC_ACQ_REL, HS_AGENT);
   const uint32_t e = (uint32_t)(old / parts);
   if ((old + weight) % parts == 0) {
     if (episodes) st_rlx(episodes, ld_rlx(episodes) + 1);
-    st_rel(gen, e + 1);
+    if (AGENT) st_rlx(gen, e + 1); else st_rel(gen, e + 1);
     return true;
   }
-  return spin_until_eq(gen, e + 1, poll_q, g);
+  return AGENT ? poll_eq(gen, e + 1, poll_q, g) : spin_until_eq(gen, e + 1, poll_q, g);
 }
 
 }  // namespace hs

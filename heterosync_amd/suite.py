@@ -48,8 +48,7 @@ def c2(dev, oh, scopes=(0, 1)):
     for name in ("atomicTreeBarrUniq", "atomicTreeBarrUniqLocalExch", "lfTreeBarrUniq",
                  "lfTreeBarrUniqLocalExch") + SRB_NAMES:
         pid = hs.prim_id(name)
-        # the SRB family is flat (release/acquire) at 4 WGs per CU
-        for sc in (scopes if pid < 4 else (1,)):
+        for sc in scopes:
             s, r = dev.syncprim(pid, 10, 4 * ncu, 1000, wg_size=256, reps=10, timeout_s=30.0,
                                 sync_scope=sc)
             ok = bool(np.array_equal(s, oh.golden(pid, 10, 4 * ncu, 1000, 10, 256, ncu, 0)))

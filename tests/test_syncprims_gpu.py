@@ -19,9 +19,9 @@ SRB = [37, 38, 39, 40, 41, 42]
 
 
 def srb_scope(prim, nw, ncu, scope):
-    """The XCD-aware form is used only where the SRB barrier is a tree."""
-    per = -(-nw // min(nw, ncu))
-    return scope if prim in (37, 38) and per > 4 else 1
+    """Every SRB form honours the requested scope: the tree runs XCD-aware,
+    the flat barriers fence-free over agent-scope data."""
+    return scope
 
 
 def check(dev, ohelp, prim, L, nw, iters, reps=10, wg=64, mode=0, literal=False, **kw):
@@ -120,11 +120,12 @@ def test_srb_small(dev, ohelp, prim, mode, scope):
 
 
 @pytest.mark.parametrize("prim", SRB)
-def test_srb_full_config(dev, ohelp, prim):
+@pytest.mark.parametrize("scope", [0, 1])
+def test_srb_full_config(dev, ohelp, prim, scope):
     """1024 WGs x 256 threads, L=10, 1000 iterations x 2 launches."""
     ncu = dev.info.num_cu
-    r = check(dev, ohelp, prim, 10, 4 * ncu, 1000, reps=2, wg=256, mode=1)
-    assert r.episodes == 2000
+    r = check(dev, ohelp, prim, 10, 4 * ncu, 1000, reps=2, wg=256, mode=1, sync_scope=scope)
+    assert r.episodes == 2000 and r.sync_scope_used == scope
 
 
 def test_fa_mutex_c1(dev, ohelp):
