@@ -144,14 +144,13 @@ __global__ void __launch_bounds__(64) k_batch(BArgs a) {
       if (lane == 0) {
         uint32_t *lk = I.lk;
         if (!sem) {
-          st_rlx(lk + kLkCs, ld_rlx(lk + kLkCs) + 1);
           if (p == 4 || p == 5) mutex_unlock<kSpin>(lk, nullptr, 0, 0);
           else if (p == 7) mutex_unlock<kFA>(lk, nullptr, 0, s_ticket[j]);
           else mutex_unlock<kSleep>(lk, I.ring, I.ring_stride, s_ticket[j]);
+          add_rlx(lk + kLkCs, 1);  // entry count, off the hand-off path
         } else {
-          if (writer) st_rlx(lk + kLkCs, ld_rlx(lk + kLkCs) + 1);
-          else add_rlx(lk + kLkRd, 1);
           sem_post(lk, writer, a.bo, g);
+          add_rlx(lk + (writer ? kLkCs : kLkRd), 1);
         }
         s_bo[j] = a.bo.min_q;
         s_phase[j] = ++s_iter[j] == I.iters ? kDone : kIdle;
