@@ -7,6 +7,7 @@
 //      WG, one per lane (the uncontended peak)
 //   5. visibility: which store/load flavours observe each other across WGs
 //      on the same XCD and across XCDs
+//   6. a line's return to an XCD whose L2 an earlier store left it in
 // Every wait is bounded; prints one line per measurement.
 #include <hip/hip_runtime.h>
 
@@ -128,6 +129,53 @@ __global__ void k_vis(int a, int b, int rounds, int wmode, int rmode, uint32_t *
   }
 }
 
+// A line's return to an XCD.  Per round, four steps handed on by a flag:
+//   0  WG a  (XCD x) stores with flavour f0 (0 plain, 2 agent-scope)
+//   1  WG a2 (XCD x) agent-loads it, then stores with flavour f1
+//   2  WG b  (XCD y) agent-loads it, then agent-stores
+//   3  WG a  (XCD x) agent-loads it
+// res[1..3] count the stale values read at steps 1..3, res[4] a timeout.
+// Step 3 reads stale exactly when XCD x's L2 still returns a line that an
+// earlier store on x left there after y's store reached memory.
+__global__ void k_return(int a, int a2, int b, int rounds, int f0, int f1, uint32_t *flag,
+                         int32_t *data, uint32_t *res) {
+  const int me = blockIdx.x;
+  if (me != a && me != a2 && me != b) return;
+  int32_t *p = data + threadIdx.x * 16;
+  bool dead = false;  // master: a wait timed out, stop waiting
+  for (int r = 1; r <= rounds; ++r) {
+    for (int step = 0; step < 4; ++step) {
+      const int who = step == 1 ? a2 : step == 2 ? b : a;
+      if (who != me) continue;
+      const uint32_t s = 4u * (uint32_t)(r - 1) + (uint32_t)step;
+      if (threadIdx.x == 0 && !dead) {
+        uint32_t n = 0;
+        while (ld_rlx(flag) != s)
+          if (++n > (1u << 22)) {
+            res[4] = 1;
+            dead = true;
+            break;
+          }
+      }
+      __syncthreads();
+      if (step > 0) {
+        const int32_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, HS_AGENT);
+        if (v != 10 * r + step) atomicAdd(res + step, 1u);
+      }
+      const int fl = step == 0 ? f0 : step == 1 ? f1 : 2;
+      if (step < 3) {
+        if (fl == 2)
+          __hip_atomic_store(p, 10 * r + step + 1, __ATOMIC_RELAXED, HS_AGENT);
+        else
+          *p = 10 * r + step + 1;
+      }
+      wait_mem();
+      __syncthreads();
+      if (threadIdx.x == 0) st_rlx(flag, s + 1);
+    }
+  }
+}
+
 // Plain streaming copy of known size: the calibration reference for the
 // FETCH_SIZE / WRITE_SIZE hardware counters.
 __global__ void k_copy(const int4 *__restrict__ src, int4 *__restrict__ dst, long n) {
@@ -233,6 +281,24 @@ int main(int argc, char **argv) {
                     pr ? "other XCD" : "same XCD ", hr[1], rounds * 256,
                     hr[3] ? " (TIMEOUT)" : "");
       }
+  {  // a line's return to an XCD: WGs 0 and 8 share an XCD, WG 1 is on another
+    static const int kF[3][2] = {{0, 0}, {0, 2}, {2, 2}};
+    static const char *kN[] = {"plain", "", "agent"};
+    for (int m = 0; m < 3; ++m) {
+      const int rounds = 2000;
+      CHECK(hipMemset(buf, 0, 4096));
+      CHECK(hipMemset(res, 0, 64));
+      CHECK(hipMemset(data, 0, 1 << 20));
+      k_return<<<16, 256>>>(0, 8, 1, rounds, kF[m][0], kF[m][1], buf, data, res);
+      CHECK(hipDeviceSynchronize());
+      uint32_t hr[5];
+      CHECK(hipMemcpy(hr, res, 20, hipMemcpyDeviceToHost));
+      std::printf("return x:%s st, x:agent ld + %s st, y:agent ld + agent st, x:agent ld: "
+                  "stale %u / %u / %u of %d%s\n",
+                  kN[kF[m][0]], kN[kF[m][1]], hr[1], hr[2], hr[3], rounds * 256,
+                  hr[4] ? " (TIMEOUT)" : "");
+    }
+  }
   std::printf("done\n");
   return 0;
 }

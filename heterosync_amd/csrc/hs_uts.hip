@@ -862,7 +862,7 @@ struct Engine {
     return HS_OK;
   }
 
-  // Clears the per-run state on the stream (a fill kernel).
+  // Clears the per-run state on the stream (never inside a captured graph).
   int reset(hipStream_t s) {
     HS_HIP(hipMemsetAsync(state, 0, state_words * 4, s));
     return HS_OK;
