@@ -117,14 +117,71 @@ __global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
   }
 }
 
+// Which WG touches which region, in the XCD-aware form.  The final storage
+// depends only on each region's sequence of passes, not on which WG makes
+// them, so the data walk is laid out for the XCDs' L2s: a pass whose region
+// is next touched by a WG on the same XCD stores with keep_l2, and only a
+// pass whose region moves to another XCD stores at agent scope.  Logical XCD
+// r = b % nxcd (the placement check makes it one physical XCD); j = b / nxcd
+// numbers the WGs of one XCD.
+//   - Without local exchange, region cur is next touched by the WG with the
+//     next lower data id.  Data ids are numbered XCD by XCD, d = r *
+//     (grid / nxcd) + j, so only the first id of each XCD hands its region to
+//     another XCD.
+//   - NaiveSRB's exchange re-visits the WG's own region: the first pass
+//     keeps, the second follows the rule above.
+//   - The group exchange (stride nab) hands the first region to d - nab and
+//     the second to d - 1 (next episode).  With gx = nab / nxcd, the ids
+//     d = nab * (j / gx) + gx * r + j % gx put d - nab on the same XCD always
+//     and d - 1 unless d % gx == 0.  The barrier groups follow the data ids
+//     (members d % nab, index d / nab, as the reference's exchange needs),
+//     renumbered so that group g lies on logical XCD g % nxcd.
+// Agent-scope stores over lines left dirty by another CU measured slow when
+// every WG makes them (a walk that did so ran 2x slower), so each walk keeps
+// them to a few WGs per XCD.  Kept stores rely on the placement check (WGs
+// with equal b % nxcd share an XCD), for the writers and for the WGs that
+// read after them; `placed` is the same for the whole grid.  In the strict
+// form data ids and groups are the reference's (d = b, group b % nab).
+struct DataPlan {
+  int id;          // data id: the first region this WG touches
+  int grp, q;      // barrier group and index within it
+  bool keep_first, keep_second;
+  bool placed;     // some WG of the grid keeps stores in L2
+};
+__device__ __forceinline__ DataPlan data_plan(bool xcd, bool lx, bool self_exchange, int b,
+                                              int grid, int nab, int nx) {
+  DataPlan d{b, b % nab, b / nab, false, false, false};
+  if (!xcd || nx <= 1 || grid % nx) return d;
+  const int r = b % nx, j = b / nx;
+  if (!lx || self_exchange) {
+    const int per_x = grid / nx;
+    d.id = r * per_x + j;
+    const bool next_here = d.id % per_x != 0;
+    d.keep_first = lx ? true : next_here;
+    d.keep_second = lx && next_here;
+    d.placed = lx || per_x > 1;
+  } else if (nab % nx == 0 && grid % nab == 0) {
+    const int gx = nab / nx;
+    d.id = nab * (j / gx) + gx * r + j % gx;
+    const int gd = d.id % nab;  // data group: members gd + k * nab, all on XCD r
+    d.grp = (gd % gx) * nx + gd / gx;
+    d.q = d.id / nab;
+    d.keep_first = true;
+    d.keep_second = d.id % gx != 0;
+    d.placed = true;
+  }
+  return d;
+}
+
 template <bool FLAG, bool LX, int MODE, bool XCD>
 __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
   __shared__ int s_ok, s_flag;
   Guard g;
   g.init(a.ctl, a.ticks);
   const int b = blockIdx.x, grid = gridDim.x, nab = a.nab;
-  const int grp = b % nab, q = b / nab;  // cuID, perCU_WGID (HM:122-132)
-  int cur = b;
+  const DataPlan dp = data_plan(XCD, LX && a.per_cu > 1, false, b, grid, nab, a.nxcd);
+  const int grp = dp.grp, q = dp.q;  // cuID, perCU_WGID (HM:122-132)
+  int cur = dp.id;
   uint32_t eg = a.e_global0, el = a.e_local0;
   if (is_master()) {
     s_ok = !g.aborted();
@@ -143,7 +200,7 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
   __syncthreads();
   if (!s_ok) return;
   for (int i = 0; i < a.iters; ++i) {
-    wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
+    wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L, dp.keep_first);
     if (LX && a.per_cu > 1) {
       bool ok;
       if (FLAG) {
@@ -160,7 +217,7 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
       }
       if (!ok) return;
       cur = (cur + nab) % grid;  // HM:355-358
-      wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
+      wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L, dp.keep_second);
     }
     bool ok;
     if (FLAG) {
@@ -201,16 +258,18 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
   Guard g;
   g.init(a.ctl, a.ticks);
   const int b = blockIdx.x, grid = gridDim.x, nab = a.nab;
-  const int grp = b % nab;
   const bool lx = (VAR & 1) && a.per_cu > 1;
   const bool all = VAR >= 4;
   const bool tree = VAR <= 1 && a.per_cu > 4;
   const uint64_t parts = all ? (uint64_t)grid * blockDim.x : (uint64_t)grid;
   const bool wave_lead = (threadIdx.x & 63) == 0;
-  int cur = b;
+  const DataPlan dp = data_plan(XCD, lx, VAR == 3, b, grid, nab, a.nxcd);
+  const int grp = dp.grp;
+  int cur = dp.id;
   if (is_master()) {
     s_ok = !g.aborted();
-    if (XCD && tree) {  // only the XCD tree depends on where WGs run
+    // the XCD tree and the L2-kept data walk depend on where WGs run
+    if (XCD && (tree || dp.placed)) {
       const uint32_t me = xcc_id();
       const uint32_t old = cas_rlx(a.xcd_map + b % a.nxcd, 0xffffffffu, me);
       if (old != 0xffffffffu && old != me) {
@@ -241,11 +300,11 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
   };
   for (int i = 0; i < a.iters; ++i) {
     epoch = a.e_global0 + i + 1;
-    wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
+    wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L, dp.keep_first);
     if (lx) {
       if (!barrier(true)) return;
       if (VAR != 3) cur = (cur + nab) % grid;
-      wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L);
+      wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L, dp.keep_second);
     }
     if (!barrier(false)) return;
     cur = (cur + 1) % grid;
@@ -340,12 +399,15 @@ extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
                                          std::min(cfg->num_wgs, ncu)
                                    : 1;
   const bool xcd_form = (p >= 0 && p < 4) || ((p == 37 || p == 38) && per > 4);
-  // mutexes, semaphores and the flat sense-reversing barriers: the fence-free
-  // (agent-scope data) form needs no placement, so it runs directly
+  // mutexes and semaphores: the fence-free (agent-scope data) form needs no
+  // placement, so it runs directly; so do the flat sense-reversing barriers
+  // on a one-XCD device.  With several XCDs their data walk keeps stores in
+  // L2 and checks the placement like the trees.
   const bool srb_flat = p >= 37 && p <= 42 && !xcd_form;
-  if (cfg->sync_scope == 0 && (is_mutex(p) || is_sem(p) || srb_flat))
+  const int nx = num_xcds(c);
+  if (cfg->sync_scope == 0 && (is_mutex(p) || is_sem(p) || (srb_flat && nx == 1)))
     return run_once(c, cfg, storage, n, res, true, nullptr);
-  const bool xcd = cfg->sync_scope == 0 && xcd_form && num_xcds(c) > 1;
+  const bool xcd = cfg->sync_scope == 0 && (xcd_form || srb_flat) && nx > 1;
   if (!xcd) return run_once(c, cfg, storage, n, res, false, nullptr);
   std::vector<int32_t> init(storage, storage + (n > 0 ? n : 0));
   bool topo = false;
@@ -415,8 +477,10 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
     return rc;
   const size_t bar_words = (size_t)kBarWords * (1 + 2 * nxcd + nab) + HS_LINE_WORDS;
   if ((rc = arena.alloc(&d_bar, bar_words))) return rc;
+  // strict: packed flag arrays; XCD: one line per flag (kFlagStride)
   const size_t flag_words =
-      2 * (size_t)nxcd + 2 * (size_t)nab + 2 * (size_t)nab * per + 4 * HS_LINE_WORDS;
+      (2 * (size_t)nxcd + 2 * (size_t)nab + 2 * (size_t)nab * per) * (xcd_mode ? kFlagStride : 1) +
+      4 * HS_LINE_WORDS;
   if ((rc = arena.alloc(&d_flags, flag_words))) return rc;
   uint32_t *d_xmap, *d_flat;
   if ((rc = arena.alloc(&d_xmap, (size_t)reps * nxcd))) return rc;
@@ -456,8 +520,12 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   uint32_t *fl = d_flags + 4 * HS_LINE_WORDS;
   a.fb = FlagBar{fl, fl + nab, fl + 2 * nab, fl + 2 * nab + (size_t)nab * per, episodes,
                  nab, per};
-  a.xf = XcdFlags{fl, fl + nxcd, fl + 2 * nxcd, fl + 2 * nxcd + nab, fl + 2 * nxcd + 2 * nab,
-                  fl + 2 * nxcd + 2 * nab + (size_t)nab * per, episodes, nab, per, nxcd};
+  {
+    const size_t S = kFlagStride;
+    a.xf = XcdFlags{fl, fl + nxcd * S, fl + 2 * nxcd * S, fl + (2 * nxcd + nab) * S,
+                    fl + (2 * nxcd + 2 * nab) * S,
+                    fl + (2 * nxcd + 2 * nab + (size_t)nab * per) * S, episodes, nab, per, nxcd};
+  }
   a.flat = d_flat;
   a.nxcd = nxcd;
   a.L = L;
@@ -606,27 +674,30 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
       std::memcpy(res->lock_words, w, sizeof(w));
     } else {
       const uint32_t *f = &flags[4 * HS_LINE_WORDS];
+      const size_t S = xcd_mode ? kFlagStride : 1;  // words between flags
       const uint32_t *lin, *lout;
       uint32_t lout_want = tl;
       if (xcd_mode) {  // single-hop release words gout[x]; out-flags carry local epochs only
-        const uint32_t *gin = f, *gout = f + nxcd, *xin = f + 2 * nxcd, *xout = xin + nab;
+        const uint32_t *gin = f, *gout = f + nxcd * S, *xin = f + 2 * nxcd * S,
+                       *xout = xin + nab * S;
         const int nx = std::min(nab, nxcd);
-        for (int x = 0; x < nx; ++x) ok = ok && gout[x] == te && (x == 0 || gin[x] == te);
-        for (int gi = nxcd; gi < nab; ++gi) ok = ok && xin[gi] == te && xout[gi] == 0;
-        lin = xout + nab;
+        for (int x = 0; x < nx; ++x) ok = ok && gout[x * S] == te && (x == 0 || gin[x * S] == te);
+        for (int gi = nxcd; gi < nab; ++gi) ok = ok && xin[gi * S] == te && xout[gi * S] == 0;
+        lin = xout + nab * S;
         lout_want = lx ? tl - 1 : 0;
       } else {
         const uint32_t *gin = f, *gout = f + nab;
         for (int gi = 1; gi < nab; ++gi) ok = ok && gin[gi] == te && gout[gi] == te;
         lin = gout + nab;
       }
-      lout = lin + (size_t)nab * per;
+      lout = lin + (size_t)nab * per * S;
       if (per > 1)
         for (int gi = 0; gi < nab; ++gi)
           for (int m = 1; m < per; ++m)
-            ok = ok && lin[(size_t)gi * per + m] == tl && lout[(size_t)gi * per + m] == lout_want;
-      const uint32_t w[8] = {f[1], f[1 + (xcd_mode ? nxcd : nab)], per > 1 ? lin[1] : 0,
-                             per > 1 ? lout[1] : 0, ep, 0, 0, 0};
+            ok = ok && lin[((size_t)gi * per + m) * S] == tl &&
+                 lout[((size_t)gi * per + m) * S] == lout_want;
+      const uint32_t w[8] = {f[S], f[(1 + (xcd_mode ? nxcd : nab)) * S], per > 1 ? lin[S] : 0,
+                             per > 1 ? lout[S] : 0, ep, 0, 0, 0};
       std::memcpy(res->lock_words, w, sizeof(w));
     }
   }

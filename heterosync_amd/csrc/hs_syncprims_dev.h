@@ -119,11 +119,19 @@ __device__ __forceinline__ void wg_access(int32_t *s, long tid0, int L) {
 // any of its stores -- exactly the per-thread top-down walk's result, since
 // a band's stores only overwrite lines its own loads have already read.
 // SC1: agent-scope accesses (L1 bypassed, stores written through).
+// keep_l2 (with SC1): the region's next access is by a WG on this XCD, which
+// stores the region again.  Plain stores then leave the lines dirty in this
+// XCD's L2, where that WG's agent-scope loads find them (a plain store stays
+// in its XCD's L2 and agent-scope loads on that XCD return it).  The last
+// store before a region's next access moves to another XCD must not use
+// keep_l2: an agent-scope store over a dirty line leaves no stale copy for a
+// later return of the region to this XCD.  Both facts are measured by the
+// synchronization probe (visibility and line-return sections).
 typedef uint32_t hs_v4u __attribute__((ext_vector_type(4)));
 constexpr int kRsrcWord3 = 0x00020000;  // raw buffer descriptor, gfx9 family
 
 template <int MODE, bool SC1 = false>
-__device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L) {
+__device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L, bool keep_l2 = false) {
   if (L <= 0) return;
   const int nt = (int)blockDim.x, t = (int)threadIdx.x;
   const int nc = nt / kWordsPerLine;  // chains in the region
@@ -150,10 +158,17 @@ __device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L) {
       wait_mem();
       __syncthreads();
     }
+    if (SC1 && !keep_l2) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (dst[k] >= 0)
-        __builtin_amdgcn_raw_buffer_store_b128(MODE == 0 ? v[k] : v[k] + 1u, rs, dst[k], 0, kAux);
+      for (int k = 0; k < 4; ++k)
+        if (dst[k] >= 0)
+          __builtin_amdgcn_raw_buffer_store_b128(MODE == 0 ? v[k] : v[k] + 1u, rs, dst[k], 0, kAux);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (dst[k] >= 0)
+          __builtin_amdgcn_raw_buffer_store_b128(MODE == 0 ? v[k] : v[k] + 1u, rs, dst[k], 0, 0);
+    }
   }
 }
 
@@ -513,6 +528,27 @@ __device__ __forceinline__ bool poll_eq(const uint32_t *p, uint32_t v,
   return true;
 }
 
+// Release waits by one lane per WG: two polls in flight.  Each check waits
+// only for the older load while the newer one is already on its way, so a
+// word that changes is seen about half a memory round trip sooner than by a
+// load-check-reload loop.  Loads of one wave return in order, so the checks
+// see the word's values in time order.  (Gathers, where many lanes of the
+// gathering WG poll at once, keep one load per lane.)
+__device__ __forceinline__ bool poll_eq_pipe(const uint32_t *p, uint32_t v,
+                                             uint32_t poll_q, Guard &g) {
+  uint32_t older = ld_rlx(p);
+  for (;;) {
+    sleep_quanta(poll_q);
+    const uint32_t newer = ld_rlx(p);
+    if (older == v) return true;
+    if (!g.ok()) return false;
+    sleep_quanta(poll_q);
+    older = ld_rlx(p);
+    if (newer == v) return true;
+    if (!g.ok()) return false;
+  }
+}
+
 __device__ __forceinline__ uint32_t xcc_id() {
   return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 0xfu;  // HW_REG_XCC_ID
 }
@@ -548,24 +584,29 @@ __device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp,
   if (local_only) {  // group-level barrier (local exchange, HM:352-353)
     if (per <= 1) return true;
     uint32_t *cnt = t.grp + (long)grp * kBarWords, *gen = cnt + 32;
-    if ((add_rlx(cnt, 1) + 1) % per != 0) return poll_eq(gen, epoch, poll_q, g);
+    if ((add_rlx(cnt, 1) + 1) % per != 0) return poll_eq_pipe(gen, epoch, poll_q, g);
     st_rlx(gen, epoch);
     return true;
   }
   const int x = grp % t.nxcd;
   uint32_t *rel = t.rel + (long)x * kBarWords;
   if (per > 1 && (add_rlx(t.grp + (long)grp * kBarWords, 1) + 1) % per != 0)
-    return poll_eq(rel, epoch, poll_q, g);
+    return poll_eq_pipe(rel, epoch, poll_q, g);
   const uint32_t ngx = (uint32_t)groups_on_xcd(t.nab, t.nxcd, x);
   if ((add_rlx(t.xcd + (long)x * kBarWords, 1) + 1) % ngx != 0)
-    return poll_eq(rel, epoch, poll_q, g);
+    return poll_eq_pipe(rel, epoch, poll_q, g);
   const uint32_t nx = (uint32_t)(t.nab < t.nxcd ? t.nab : t.nxcd);
-  if ((add_rlx(t.glob, 1) + 1) % nx != 0) return poll_eq(rel, epoch, poll_q, g);
+  if ((add_rlx(t.glob, 1) + 1) % nx != 0) return poll_eq_pipe(rel, epoch, poll_q, g);
   st_rlx(t.episodes, ld_rlx(t.episodes) + 1);
   st_rlx(t.glob + 32, epoch);
   for (uint32_t y = 0; y < nx; ++y) st_rlx(t.rel + (long)y * kBarWords, epoch);
   return true;
 }
+
+// Flag i of an array lives at word i * kFlagStride: one 128-byte line per
+// flag, so the WGs that store flags and the gatherers that poll them never
+// share a line.
+constexpr int kFlagStride = HS_LINE_WORDS;
 
 struct XcdFlags {
   uint32_t *gin, *gout;  // nxcd: device level (XCD leaders)
@@ -587,25 +628,27 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
                                                  bool local_only, uint32_t poll_q,
                                                  Guard &g, int *s_flag) {
   const int tid = threadIdx.x, nthr = blockDim.x;
+  constexpr long S = kFlagStride;
   const int x = grp % f.nxcd;
-  const uint32_t *rel = f.gout + x;
+  const uint32_t *rel = f.gout + x * S;
   wait_mem();
   __syncthreads();
   if (f.per_cu > 1) {
-    uint32_t *lin = f.lin + (long)grp * f.per_cu, *lout = f.lout + (long)grp * f.per_cu;
+    uint32_t *lin = f.lin + (long)grp * f.per_cu * S, *lout = f.lout + (long)grp * f.per_cu * S;
     if (q != 0) {
       if (tid == 0) {
-        st_rlx(lin + q, el);
-        *s_flag = local_only ? poll_eq(lout + q, el, poll_q, g) : poll_eq(rel, eg, poll_q, g);
+        st_rlx(lin + q * S, el);
+        *s_flag = local_only ? poll_eq_pipe(lout + q * S, el, poll_q, g)
+                             : poll_eq_pipe(rel, eg, poll_q, g);
       }
       __syncthreads();
       return *s_flag != 0;
     }
     bool ok = true;
-    for (int m = 1 + tid; m < f.per_cu; m += nthr) ok = ok && poll_eq(lin + m, el, poll_q, g);
+    for (int m = 1 + tid; m < f.per_cu; m += nthr) ok = ok && poll_eq(lin + m * S, el, poll_q, g);
     if (!__syncthreads_and(ok)) return false;
     if (local_only) {
-      for (int m = 1 + tid; m < f.per_cu; m += nthr) st_rlx(lout + m, el);
+      for (int m = 1 + tid; m < f.per_cu; m += nthr) st_rlx(lout + m * S, el);
       return true;
     }
   } else if (local_only) {
@@ -614,8 +657,8 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
   const int k = grp / f.nxcd;
   if (k != 0) {  // group master: arrive at the XCD level, wait for the release
     if (tid == 0) {
-      st_rlx(f.xin + grp, eg);
-      *s_flag = poll_eq(rel, eg, poll_q, g);
+      st_rlx(f.xin + grp * S, eg);
+      *s_flag = poll_eq_pipe(rel, eg, poll_q, g);
     }
     __syncthreads();
     return *s_flag != 0;
@@ -623,13 +666,14 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
   // XCD leader: gather the XCD's group masters, one flag per lane
   const int ngx = groups_on_xcd(f.nab, f.nxcd, x);
   bool ok = true;
-  for (int j = 1 + tid; j < ngx; j += nthr) ok = ok && poll_eq(f.xin + x + j * f.nxcd, eg, poll_q, g);
+  for (int j = 1 + tid; j < ngx; j += nthr)
+    ok = ok && poll_eq(f.xin + (x + (long)j * f.nxcd) * S, eg, poll_q, g);
   if (!__syncthreads_and(ok)) return false;
   const int nx = f.nab < f.nxcd ? f.nab : f.nxcd;
   if (x != 0) {  // XCD leader: arrive at the device level, wait for the release
     if (tid == 0) {
-      st_rlx(f.gin + x, eg);
-      *s_flag = poll_eq(rel, eg, poll_q, g);
+      st_rlx(f.gin + x * S, eg);
+      *s_flag = poll_eq_pipe(rel, eg, poll_q, g);
     }
     __syncthreads();
     return *s_flag != 0;
@@ -637,10 +681,10 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
   // device master: gather the other XCD leaders, one flag per lane, then
   // release every XCD
   ok = true;
-  for (int y = 1 + tid; y < nx; y += nthr) ok = ok && poll_eq(f.gin + y, eg, poll_q, g);
+  for (int y = 1 + tid; y < nx; y += nthr) ok = ok && poll_eq(f.gin + y * S, eg, poll_q, g);
   if (!__syncthreads_and(ok)) return false;
   if (tid == 0) st_rlx(f.episodes, ld_rlx(f.episodes) + 1);
-  for (int y = tid; y < nx; y += nthr) st_rlx(f.gout + y, eg);
+  for (int y = tid; y < nx; y += nthr) st_rlx(f.gout + y * S, eg);
   return true;
 }
 
@@ -671,7 +715,7 @@ __device__ __forceinline__ bool flat_arrive(uint32_t *blk, uint32_t weight, uint
     if (AGENT) st_rlx(gen, e + 1); else st_rel(gen, e + 1);
     return true;
   }
-  return AGENT ? poll_eq(gen, e + 1, poll_q, g) : spin_until_eq(gen, e + 1, poll_q, g);
+  return AGENT ? poll_eq_pipe(gen, e + 1, poll_q, g) : spin_until_eq(gen, e + 1, poll_q, g);
 }
 
 }  // namespace hs
