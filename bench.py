@@ -24,13 +24,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
-# HBM bytes per barrier episode of k_barrier (atomicTreeBarrUniq, XCD-aware,
-# 1024 WGs x 256 lanes, numLdSt=10), from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE per dispatch at numCSIters=100, each scaled by what the same
-# counter reads for a 256 MiB copy kernel (FETCH_SIZE x 2.000, WRITE_SIZE
-# x 1.000); the command lines and raw figures are in
-# profiles/k_barrier_atomicTreeBarrUniq_1024x256.txt.
-MEASURED_TRAFFIC = {("atomicTreeBarrUniq", 0, 256, 10): 22063424}
+# HBM bytes per launch of k_barrier (atomicTreeBarrUniq, XCD-aware, 1024 WGs
+# x 256 lanes, numLdSt=10, numCSIters=1000), from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE per dispatch, each scaled by what the same counter reads for
+# a 256 MiB copy kernel (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the command
+# lines and raw figures are in profiles/k_barrier_atomicTreeBarrUniq_1024x256.txt.
+# Key: (prim, sync scope, WG size, numLdSt, numCSIters, num WGs).
+MEASURED_TRAFFIC = {("atomicTreeBarrUniq", 0, 256, 10, 1000, 1024): 392554598}
 METRIC = ("acquire+release/s & barrier episodes/s at 1 GPU, batched vs plain; "
           "UTS nodes/s, p99")
 
@@ -149,9 +149,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": (MEASURED_TRAFFIC[key] * a.iters
-                        if (key := (a.prim, scope, a.wg, a.num_ldst)) in MEASURED_TRAFFIC
-                        and nw == 1024 else None),
+            "traffic": MEASURED_TRAFFIC.get((a.prim, scope, a.wg, a.num_ldst, a.iters, nw)),
             "kernel": f"k_barrier ({a.prim})",
             "avg_launch_ms": round(avg_launch_ms, 4),
             "bytes_per_episode": bytes_per_episode,
