@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
         if (XCD) wait_mem();
         __syncthreads();
         if (is_master())
-          s_ok = XCD ? xcd_tree_arrive(a.xt, grp, true, a.e_global0 + i + 1, a.bo.poll_q, g)
+          s_ok = XCD ? xcd_tree_arrive(a.xt, grp, q, true, a.e_global0 + i + 1, a.bo.poll_q, g)
                      : atomic_tree_arrive(a.tb, grp, true, a.bo.poll_q, g);
         __syncthreads();
         ok = s_ok;
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
       if (XCD) wait_mem();
       __syncthreads();
       if (is_master())
-        s_ok = XCD ? xcd_tree_arrive(a.xt, grp, false, a.e_global0 + i + 1, a.bo.poll_q, g)
+        s_ok = XCD ? xcd_tree_arrive(a.xt, grp, q, false, a.e_global0 + i + 1, a.bo.poll_q, g)
                    : atomic_tree_arrive(a.tb, grp, false, a.bo.poll_q, g);
       __syncthreads();
       ok = s_ok;
@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
     bool ok = true;
     if ((local && VAR == 1) || (!local && tree)) {
       if (is_master())
-        ok = XCD ? xcd_tree_arrive(a.xt, grp, local, epoch, a.bo.poll_q, g)
+        ok = XCD ? xcd_tree_arrive(a.xt, grp, dp.q, local, epoch, a.bo.poll_q, g)
                  : atomic_tree_arrive(a.tb, grp, local, a.bo.poll_q, g);
     } else {
       uint32_t *blk = a.flat + (local ? kBarWords : 0);
@@ -475,7 +475,18 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   if ((rc = arena.alloc(&d_locks, (size_t)nlocks * kLockWords))) return rc;
   if (sleep_mx && (rc = arena.alloc(&d_ring, (size_t)nlocks * ring_len * ring_stride)))
     return rc;
-  const size_t bar_words = (size_t)kBarWords * (1 + 2 * nxcd + nab) + HS_LINE_WORDS;
+  // barrier words, kBarWords blocks: [global][XCD-aware sub-counters:
+  // nxcd x sub_max][groups: nab][release words: nxcd][episode count line]
+  int sub_max = 0, n_sub = 0;  // XCD-aware tree: sub-counters per XCD, in all
+  if (xcd_mode)
+    for (int x = 0; x < nxcd; ++x) {
+      const int nwx = (nab / nxcd + (x < nab % nxcd ? 1 : 0)) * per;  // WGs on XCD x
+      sub_max = std::max(sub_max, (nwx + kSubArrive - 1) / kSubArrive);
+      n_sub += (nwx + kSubArrive - 1) / kSubArrive;
+    }
+  const size_t o_sub = kBarWords, o_grp = o_sub + (size_t)kBarWords * nxcd * sub_max;
+  const size_t o_rel = o_grp + (size_t)kBarWords * nab, o_ep = o_rel + (size_t)kBarWords * nxcd;
+  const size_t bar_words = o_ep + HS_LINE_WORDS;
   if ((rc = arena.alloc(&d_bar, bar_words))) return rc;
   // strict: packed flag arrays; XCD: one line per flag (kFlagStride)
   const size_t flag_words =
@@ -511,12 +522,12 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   a.ring = d_ring;
   a.ring_len = ring_len;
   a.ring_stride = ring_stride;
-  uint32_t *episodes = d_bar + kBarWords * (1 + 2 * nxcd + nab);
+  uint32_t *episodes = d_bar + o_ep;
   // strict: global block, nab group blocks;
   // XCD: global, nxcd XCD blocks, nab group blocks, nxcd release words
-  a.tb = TreeBar{d_bar, d_bar + kBarWords, episodes, nab, per};
-  a.xt = XcdTree{d_bar, d_bar + kBarWords, d_bar + kBarWords * (1 + nxcd),
-                 d_bar + kBarWords * (1 + nxcd + nab), episodes, nab, per, nxcd};
+  a.tb = TreeBar{d_bar, d_bar + o_grp, episodes, nab, per};
+  a.xt = XcdTree{d_bar, d_bar + o_sub, d_bar + o_grp, d_bar + o_rel, episodes, nab, per, nxcd,
+                 sub_max, n_sub};
   uint32_t *fl = d_flags + 4 * HS_LINE_WORDS;
   a.fb = FlagBar{fl, fl + nab, fl + 2 * nab, fl + 2 * nab + (size_t)nab * per, episodes,
                  nab, per};
@@ -602,7 +613,8 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
       if (is_sem(p) && uniq) wgs_here = (l < nab) ? per : 0;
       const uint64_t acq = (uint64_t)wgs_here * iters * reps;
       if (p == 4 || p == 5 || p == 20 || p == 21) ok = ok && lk[kLkHead] == 0;
-      if (p == 7 || p == 23) ok = ok && lk[kLkHead] == (uint32_t)acq && lk[kLkTail] == (uint32_t)acq;
+      if (p == 7 || p == 23)
+        ok = ok && lk[kLkHead] == (uint32_t)acq && lk[kLkTail] == (uint32_t)acq;
       if (sleep_mx) {
         ok = ok && lk[kLkTail] == (uint32_t)acq;
         if (acq + 1 <= (uint64_t)ring_len) {
@@ -626,7 +638,7 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
                            lk0[kLkTail], lk0[kLkCs], lk0[kLkRd], (uint32_t)nlocks};
     std::memcpy(res->lock_words, w, sizeof(w));
   } else {
-    const uint32_t ep = bar[kBarWords * (1 + 2 * nxcd + nab)];
+    const uint32_t ep = bar[o_ep];
     res->episodes = ep;
     ok = ok && ep == (uint32_t)total_ep;
     const uint32_t te = (uint32_t)total_ep, tl = (uint32_t)(total_ep * local_per_iter);
@@ -642,7 +654,7 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
       if ((p == 40 || p == 42) && per > 1) ok = ok && flat_ok(1);
       if (p == 38 && per > 1) {  // per-group local barrier
         // strict: counter reset each episode; fence-free: never reset
-        const size_t g0 = xcd_mode ? kBarWords * (1 + nxcd) : kBarWords;
+        const size_t g0 = o_grp;
         const uint32_t cnt_want = xcd_mode ? te * (uint32_t)per : 0u;
         for (int gi = 0; gi < nab; ++gi)
           ok = ok && bar[g0 + kBarWords * gi] == cnt_want && bar[g0 + kBarWords * gi + 32] == te;
@@ -650,19 +662,22 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
       const uint32_t w[8] = {flat[0], flat[1], flat[32], flat[kBarWords + 32], ep, 0, 0, 0};
       std::memcpy(res->lock_words, w, sizeof(w));
     } else if (p <= 1 || srb) {
-      const size_t g0 = xcd_mode ? kBarWords * (1 + nxcd) : kBarWords;
+      const size_t g0 = o_grp;
       if (xcd_mode) {
-        // counters are never reset: each ends at episodes x level size
+        // counters are never reset: each ends at episodes x its arrivals
         const int nx = std::min(nab, nxcd);
-        ok = ok && bar[0] == te * (uint32_t)nx && bar[32] == te;
+        ok = ok && bar[0] == te * (uint32_t)n_sub && bar[32] == te;
         for (int x = 0; x < nx; ++x) {
-          const uint32_t ngx = (uint32_t)((nab - x + nxcd - 1) / nxcd);
-          ok = ok && bar[kBarWords * (1 + x)] == te * ngx &&
-               bar[kBarWords * (1 + nxcd + nab + x)] == te;  // release word
+          const int nwx = (nab / nxcd + (x < nab % nxcd ? 1 : 0)) * per;
+          for (int k = 0; k < sub_max; ++k) {
+            const int ck = std::max(0, std::min(kSubArrive, nwx - k * kSubArrive));
+            ok = ok && bar[o_sub + (size_t)kBarWords * (x * sub_max + k)] == te * (uint32_t)ck;
+          }
+          ok = ok && bar[o_rel + (size_t)kBarWords * x] == te;  // release word
         }
-        if (per > 1)
+        if (per > 1)  // groups: local-exchange arrivals only
           for (int gi = 0; gi < nab; ++gi)
-            ok = ok && bar[g0 + kBarWords * gi] == te * (uint32_t)per * (lx ? 2u : 1u) &&
+            ok = ok && bar[g0 + kBarWords * gi] == (lx ? te * (uint32_t)per : 0u) &&
                  bar[g0 + kBarWords * gi + 32] == (lx ? te : 0u);
       } else {
         ok = ok && bar[0] == 0 && bar[32] == te;

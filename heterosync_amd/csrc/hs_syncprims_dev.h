@@ -557,27 +557,36 @@ __device__ __forceinline__ int groups_on_xcd(int nab, int nxcd, int x) {
   return nab / nxcd + (x < nab % nxcd ? 1 : 0);
 }
 
+// WGs per arrival counter of the XCD-aware tree: 16 balances the two
+// levels' serialisation at one address (about 11 ns per RMW) against one
+// more dependent round trip per level (1024 WGs: 64 counters of 16, then
+// one device counter of 64 arrivals).
+constexpr int kSubArrive = 16;
+
 struct XcdTree {
-  uint32_t *glob;      // kBarWords: count @0, gen @32 (one arrival per XCD)
-  uint32_t *xcd;       // nxcd * kBarWords: count @0, gen @32
+  uint32_t *glob;      // kBarWords: count @0, gen @32 (one arrival per sub-counter)
+  uint32_t *sub;       // nxcd * sub_max * kBarWords: sub-counters, XCD-major
   uint32_t *grp;       // nab * kBarWords: count @0, gen @32 (local exchange)
   uint32_t *rel;       // nxcd * kBarWords: per-XCD copy of the device generation
   uint32_t *episodes;
   int nab, per_cu, nxcd;
+  int sub_max, n_sub;  // sub-counters per XCD (at most) and in all
 };
 
 // Master thread; every wave of the WG has run wait_mem() + __syncthreads().
-// Arrival climbs group -> XCD -> device with relaxed RMWs on counters that
-// are never reset (the last arriver of a level is the one whose increment
-// completes a multiple of the level's size); the device releaser writes one
-// release word per XCD and every WG of that XCD polls its word, so the
-// release is a single hop.  `epoch` is the number of barriers of this kind
-// completed once this one is: the value the release word (global) or the
-// group's generation (local_only) must reach, so nothing is read before
-// arriving.  The WGs' data accesses are agent-scope (written through, L1
-// bypassed) and complete before they arrive, so no level needs a cache
-// fence.
-__device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp,
+// A global arrival climbs two levels of relaxed RMWs on counters that are
+// never reset (the last arriver of a level is the one whose increment
+// completes a multiple of the level's size): the WG's sub-counter (up to
+// kSubArrive WGs of one XCD), then the device counter (one arrival per
+// sub-counter).  The device releaser writes one release word per XCD and
+// every WG of that XCD polls its word, so the release is a single hop.
+// `q` is the WG's index in its group, so (grp / nxcd, q) numbers the WGs of
+// one XCD.  `epoch` is the number of barriers of this kind completed once
+// this one is: the value the release word (global) or the group's
+// generation (local_only) must reach, so nothing is read before arriving.
+// The WGs' data accesses are agent-scope where they leave the XCD and
+// complete before they arrive, so no level needs a cache fence.
+__device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp, int q,
                                                 bool local_only, uint32_t epoch,
                                                 uint32_t poll_q, Guard &g) {
   const uint32_t per = (uint32_t)t.per_cu;
@@ -590,13 +599,14 @@ __device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp,
   }
   const int x = grp % t.nxcd;
   uint32_t *rel = t.rel + (long)x * kBarWords;
-  if (per > 1 && (add_rlx(t.grp + (long)grp * kBarWords, 1) + 1) % per != 0)
-    return poll_eq_pipe(rel, epoch, poll_q, g);
-  const uint32_t ngx = (uint32_t)groups_on_xcd(t.nab, t.nxcd, x);
-  if ((add_rlx(t.xcd + (long)x * kBarWords, 1) + 1) % ngx != 0)
-    return poll_eq_pipe(rel, epoch, poll_q, g);
+  const int nwx = groups_on_xcd(t.nab, t.nxcd, x) * (int)per;  // WGs on this XCD
+  const int wx = (grp / t.nxcd) * (int)per + q;
+  const int k = wx / kSubArrive;
+  const int ck = nwx - k * kSubArrive < kSubArrive ? nwx - k * kSubArrive : kSubArrive;
+  uint32_t *sc = t.sub + (long)(x * t.sub_max + k) * kBarWords;
+  if ((add_rlx(sc, 1) + 1) % (uint32_t)ck != 0) return poll_eq_pipe(rel, epoch, poll_q, g);
+  if ((add_rlx(t.glob, 1) + 1) % (uint32_t)t.n_sub != 0) return poll_eq_pipe(rel, epoch, poll_q, g);
   const uint32_t nx = (uint32_t)(t.nab < t.nxcd ? t.nab : t.nxcd);
-  if ((add_rlx(t.glob, 1) + 1) % nx != 0) return poll_eq_pipe(rel, epoch, poll_q, g);
   st_rlx(t.episodes, ld_rlx(t.episodes) + 1);
   st_rlx(t.glob + 32, epoch);
   for (uint32_t y = 0; y < nx; ++y) st_rlx(t.rel + (long)y * kBarWords, epoch);

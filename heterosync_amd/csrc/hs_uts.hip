@@ -267,8 +267,9 @@ __device__ void dispatch(const UtsArgs &a, Guard &g) {
   }
 }
 
+// The work-stealing worker (one wave per WG); the kernels below wrap it.
 template <int TREE, bool SERVE>
-__global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
+__device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   constexpr int W = Rec<TREE>::W;
   __shared__ uint32_t s_pst[W * kWave];   // parent records of the batch
   __shared__ uint32_t s_top[W * kWave];   // newest children, word-major, not in the deque
@@ -746,6 +747,19 @@ __global__ void __launch_bounds__(64) k_uts(UtsArgs a) {
   }
 }
 
+// One tree per launch.  The register budget admits three waves per SIMD
+// (168 VGPRs, no spills): a grid of that size hides the record loads of
+// one wave behind the SHA-1 work of the others on big trees.
+template <int TREE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+k_uts(UtsArgs a) {
+  uts_worker<TREE, false>(a);
+}
+
+// The persistent serving loop runs two waves per SIMD (small trees gain
+// nothing from a bigger grid), so it keeps its full register allocation.
+__global__ void __launch_bounds__(64) k_uts_serve(UtsArgs a) { uts_worker<1, true>(a); }
+
 __global__ void k_root_digest(uint32_t seed, uint32_t *out) {
   uint32_t st[5];
   uts_root(seed, st);
@@ -801,13 +815,13 @@ struct Engine {
     return tree == t && blocks == nb && dq_log2 == dq && (t == 0 || b0 == b);
   }
 
-  // tree: 0 reference, 1 geometric
+  // tree: 0 reference, 1 geometric; serve: the persistent serving kernel
   int setup(hs_ctx *c, int tree_, int num_blocks, int dq_log2_, double b0_) {
     std::memset(&a, 0, sizeof a);
     tree = tree_;
     dq_log2 = dq_log2_;
     b0 = b0_;
-    blocks = num_blocks > 0 ? num_blocks : default_blocks(c);
+    blocks = num_blocks > 0 ? num_blocks : 8 * c->num_cu;
     W = tree == 1 ? 6 : 2;
     a.dq_log2 = dq_log2;
     // one inbox holds at most a quarter deque, so a late delivery fits
@@ -873,7 +887,9 @@ struct Engine {
     uint32_t res[kResultWords];
     HS_HIP(hipMemcpyAsync(res, results, sizeof res, hipMemcpyDeviceToHost, s));
     HS_HIP(hipStreamSynchronize(s));
-    auto u64 = [&](int k) { return (unsigned long long)res[2 * k] | ((unsigned long long)res[2 * k + 1] << 32); };
+    auto u64 = [&](int k) {
+      return (unsigned long long)res[2 * k] | ((unsigned long long)res[2 * k + 1] << 32);
+    };
     const uint32_t *ctl = res + 7 * kMaxSlots;
     r->nodes = u64(slot);
     r->leaves = u64(kMaxSlots + slot);
@@ -884,7 +900,18 @@ struct Engine {
     return ctl[0] ? HS_ETIMEOUT : HS_OK;
   }
 
-  static int default_blocks(hs_ctx *c) { return 8 * c->num_cu; }
+  // One geometric tree: 8 one-wave WGs per CU, or as many as can be resident
+  // (three per SIMD) when the tree is expected to be big.  Measured on
+  // MI355X (b0 = 4): the full grid wins from about 1e9 nodes (depth 14:
+  // 32.0 vs 28.7 G nodes/s; depth 15: 37.2 vs 31.3) and loses below (depth
+  // 12: 11.8 vs 14.7), where more WGs only add idle polling.
+  static int default_blocks(hs_ctx *c, double b0, int gen_mx) {
+    if (std::pow(b0, (double)gen_mx) < 2.5e8) return 8 * c->num_cu;
+    int fit = 0;
+    if (coresident_wgs(c, (const void *)k_uts<1>, 64, &fit) != HS_OK || fit <= 0)
+      return 8 * c->num_cu;
+    return fit;
+  }
 };
 
 // Root of a run, delivered to WG 0's inbox with its credit: the geometric
@@ -962,7 +989,7 @@ int run_single(hs_ctx *c, Engine &e, uint32_t seed, uint32_t gen_mx, hs_uts_resu
   HS_HIP(hipEventCreate(&e0));
   HS_HIP(hipEventCreate(&e1));
   HS_HIP(hipEventRecord(e0, c->stream));
-  hipLaunchKernelGGL((k_uts<TREE, false>), dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
+  hipLaunchKernelGGL((k_uts<TREE>), dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
   HS_HIP(hipGetLastError());
   HS_HIP(hipEventRecord(e1, c->stream));
   HS_HIP(hipEventSynchronize(e1));
@@ -1016,7 +1043,7 @@ extern "C" int hs_uts_geo_run(hs_ctx *c, int seed, double b0, int gen_mx, int nu
     return HS_EINVAL;
   std::memset(res, 0, sizeof *res);
   HS_HIP(hipSetDevice(c->device));
-  const int nb = num_blocks > 0 ? num_blocks : Engine::default_blocks(c);
+  const int nb = num_blocks > 0 ? num_blocks : Engine::default_blocks(c, b0, gen_mx);
   Engine *e = static_cast<Engine *>(c->uts_cache);
   if (!e || !e->matches(1, nb, 16, b0)) {
     if (c->uts_cache_free) c->uts_cache_free(c->uts_cache);
@@ -1045,22 +1072,30 @@ extern "C" int hs_uts_serve(hs_ctx *c, const hs_uts_request *req, int n, const h
   std::memset(resp, 0, sizeof(*resp) * n);
   const int slots = cfg->slots > 0 ? std::min(cfg->slots, kMaxSlots) : 8;
   HS_HIP(hipSetDevice(c->device));
+  int rc;
+  {  // persistent: every WG, the dispatcher (the grid's last) included, must be resident
+    int fit = 0;
+    if ((rc = coresident_wgs(c, (const void *)k_uts_serve, 64, &fit))) return rc;
+    if (cfg->num_blocks > fit) return HS_EOCCUPANCY;
+  }
   Engine e;
-  int rc = e.setup(c, 1, cfg->num_blocks, 16, b0);
-  if (rc) return rc;
+  if ((rc = e.setup(c, 1, cfg->num_blocks, 16, b0))) return rc;
   if (cfg->timeout_s > 0) e.a.ticks = timeout_ticks(c, cfg->timeout_s);
   // pinned, host-coherent request inbox / response board / stop word
-  uint32_t *h_inbox = nullptr, *h_ctl = nullptr, *h_resp = nullptr;
-  HS_HIP(hipHostMalloc((void **)&h_inbox, (size_t)n * 16, hipHostMallocCoherent | hipHostMallocMapped));
-  HS_HIP(hipHostMalloc((void **)&h_ctl, 256, hipHostMallocCoherent | hipHostMallocMapped));
-  HS_HIP(hipHostMalloc((void **)&h_resp, (size_t)kMaxSlots * 32, hipHostMallocCoherent | hipHostMallocMapped));
+  struct Pinned {
+    void *p[3] = {nullptr, nullptr, nullptr};
+    ~Pinned() { for (void *q : p) if (q) (void)hipHostFree(q); }
+  } pinned;
+  const unsigned kPinFlags = hipHostMallocCoherent | hipHostMallocMapped;
+  HS_HIP(hipHostMalloc(&pinned.p[0], (size_t)n * 16, kPinFlags));
+  HS_HIP(hipHostMalloc(&pinned.p[1], 256, kPinFlags));
+  HS_HIP(hipHostMalloc(&pinned.p[2], (size_t)kMaxSlots * 32, kPinFlags));
+  uint32_t *h_inbox = static_cast<uint32_t *>(pinned.p[0]);
+  uint32_t *h_ctl = static_cast<uint32_t *>(pinned.p[1]);
+  uint32_t *h_resp = static_cast<uint32_t *>(pinned.p[2]);
   std::memset(h_inbox, 0, (size_t)n * 16);
   std::memset(h_ctl, 0, 256);
   std::memset(h_resp, 0, (size_t)kMaxSlots * 32);
-  struct Pinned {
-    void *p[3];
-    ~Pinned() { for (void *q : p) (void)hipHostFree(q); }
-  } pinned{{h_inbox, h_ctl, h_resp}};
   volatile uint32_t *submitted = h_ctl, *stop = h_ctl + 32;
   e.a.inbox = h_inbox;
   e.a.submitted = h_ctl;
@@ -1080,7 +1115,7 @@ extern "C" int hs_uts_serve(hs_ctx *c, const hs_uts_request *req, int n, const h
   std::vector<double> lat(n, 0);
   int next = 0, done = 0, live = 0;
   const auto t0 = clk::now();
-  hipLaunchKernelGGL((k_uts<1, true>), dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
+  hipLaunchKernelGGL(k_uts_serve, dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
   if (hipGetLastError() != hipSuccess) return HS_EHIP;
   auto now_s = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
   const double deadline = cfg->timeout_s > 0 ? cfg->timeout_s : 600.0;

@@ -66,3 +66,14 @@ def test_uts_serve_alone_equals_mixed(dev, oracle):
         assert (alone.nodes, alone.leaves, alone.max_depth) == (r.nodes, r.leaves, r.max_depth)
     assert summ.total_nodes == sum(r.nodes for r in resp)
     assert 0 < summ.p50_s <= summ.p99_s
+
+
+def test_uts_serve_rejects_grid_that_cannot_be_resident(dev):
+    """The serving loop is persistent (its dispatcher is the grid's last WG),
+    so a grid larger than what can be co-resident is refused before launch."""
+    import heterosync_amd as hs
+    with pytest.raises(hs.HSError) as ei:
+        dev.uts_serve([(1, 3, 4.0, 0.0)], slots=2, num_blocks=64 * dev.info.num_cu)
+    assert ei.value.code == -4  # HS_EOCCUPANCY
+    resp, _ = dev.uts_serve([(1, 3, 4.0, 0.0)], slots=2)  # the default grid fits
+    assert resp[0].nodes > 0
