@@ -489,9 +489,14 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   const size_t bar_words = o_ep + HS_LINE_WORDS;
   if ((rc = arena.alloc(&d_bar, bar_words))) return rc;
   // strict: packed flag arrays; XCD: one line per flag (kFlagStride)
-  const size_t flag_words =
-      (2 * (size_t)nxcd + 2 * (size_t)nab + 2 * (size_t)nab * per) * (xcd_mode ? kFlagStride : 1) +
-      4 * HS_LINE_WORDS;
+  // flags: strict [gin: nab][gout: nab][lin, lout: nab x per], packed;
+  // XCD-aware [gout: nxcd][din: nxcd x sub_max][sin: nxcd x sub_max x
+  // kSubArrive][lin, lout: nab x per], one line per flag
+  const size_t n_sub_slots = (size_t)nxcd * sub_max;
+  const size_t x_flags = nxcd + n_sub_slots * (1 + kSubArrive) + 2 * (size_t)nab * per;
+  const size_t flag_words = (xcd_mode ? x_flags * kFlagStride
+                                      : 2 * (size_t)nab + 2 * (size_t)nab * per) +
+                            4 * HS_LINE_WORDS;
   if ((rc = arena.alloc(&d_flags, flag_words))) return rc;
   uint32_t *d_xmap, *d_flat;
   if ((rc = arena.alloc(&d_xmap, (size_t)reps * nxcd))) return rc;
@@ -532,10 +537,10 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   a.fb = FlagBar{fl, fl + nab, fl + 2 * nab, fl + 2 * nab + (size_t)nab * per, episodes,
                  nab, per};
   {
-    const size_t S = kFlagStride;
-    a.xf = XcdFlags{fl, fl + nxcd * S, fl + 2 * nxcd * S, fl + (2 * nxcd + nab) * S,
-                    fl + (2 * nxcd + 2 * nab) * S,
-                    fl + (2 * nxcd + 2 * nab + (size_t)nab * per) * S, episodes, nab, per, nxcd};
+    const size_t S = kFlagStride, o_din = nxcd, o_sin = o_din + n_sub_slots;
+    const size_t o_lin = o_sin + n_sub_slots * kSubArrive, o_lout = o_lin + (size_t)nab * per;
+    a.xf = XcdFlags{fl, fl + o_din * S, fl + o_sin * S, fl + o_lin * S, fl + o_lout * S,
+                    episodes, nab, per, nxcd, sub_max};
   }
   a.flat = d_flat;
   a.nxcd = nxcd;
@@ -691,28 +696,39 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
       const uint32_t *f = &flags[4 * HS_LINE_WORDS];
       const size_t S = xcd_mode ? kFlagStride : 1;  // words between flags
       const uint32_t *lin, *lout;
-      uint32_t lout_want = tl;
-      if (xcd_mode) {  // single-hop release words gout[x]; out-flags carry local epochs only
-        const uint32_t *gin = f, *gout = f + nxcd * S, *xin = f + 2 * nxcd * S,
-                       *xout = xin + nab * S;
+      uint32_t lin_want = tl, lout_want = tl;
+      uint32_t w1 = 0;
+      if (xcd_mode) {
+        // release words gout[x]; arrival flags at the sub and device levels
+        // carry episodes; group flags carry local epochs (local exchange only)
+        const uint32_t *gout = f, *din = f + nxcd * S, *sinf = din + n_sub_slots * S;
         const int nx = std::min(nab, nxcd);
-        for (int x = 0; x < nx; ++x) ok = ok && gout[x * S] == te && (x == 0 || gin[x * S] == te);
-        for (int gi = nxcd; gi < nab; ++gi) ok = ok && xin[gi * S] == te && xout[gi * S] == 0;
-        lin = xout + nab * S;
-        lout_want = lx ? tl - 1 : 0;
+        for (int x = 0; x < nx; ++x) {
+          ok = ok && gout[x * S] == te;
+          const int nwx = (nab / nxcd + (x < nab % nxcd ? 1 : 0)) * per;
+          for (int k = 0; k * kSubArrive < nwx; ++k) {
+            const size_t sid = (size_t)x * sub_max + k;
+            if (sid != 0) ok = ok && din[sid * S] == te;
+            const int ck = std::min(kSubArrive, nwx - k * kSubArrive);
+            for (int m = 1; m < ck; ++m) ok = ok && sinf[(sid * kSubArrive + m) * S] == te;
+          }
+        }
+        lin = sinf + n_sub_slots * kSubArrive * S;
+        lin_want = lout_want = lx ? tl - 1 : 0;
+        w1 = n_sub_slots > 1 ? din[S] : 0;
       } else {
         const uint32_t *gin = f, *gout = f + nab;
         for (int gi = 1; gi < nab; ++gi) ok = ok && gin[gi] == te && gout[gi] == te;
         lin = gout + nab;
+        w1 = f[1 + nab];
       }
       lout = lin + (size_t)nab * per * S;
       if (per > 1)
         for (int gi = 0; gi < nab; ++gi)
           for (int m = 1; m < per; ++m)
-            ok = ok && lin[((size_t)gi * per + m) * S] == tl &&
+            ok = ok && lin[((size_t)gi * per + m) * S] == lin_want &&
                  lout[((size_t)gi * per + m) * S] == lout_want;
-      const uint32_t w[8] = {f[S], f[(1 + (xcd_mode ? nxcd : nab)) * S], per > 1 ? lin[S] : 0,
-                             per > 1 ? lout[S] : 0, ep, 0, 0, 0};
+      const uint32_t w[8] = {f[S], w1, per > 1 ? lin[S] : 0, per > 1 ? lout[S] : 0, ep, 0, 0, 0};
       std::memcpy(res->lock_words, w, sizeof(w));
     }
   }

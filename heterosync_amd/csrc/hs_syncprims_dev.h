@@ -619,37 +619,40 @@ __device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp, int q
 constexpr int kFlagStride = HS_LINE_WORDS;
 
 struct XcdFlags {
-  uint32_t *gin, *gout;  // nxcd: device level (XCD leaders)
-  uint32_t *xin, *xout;  // nab : XCD level (group masters), index = group
-  uint32_t *lin, *lout;  // nab * per_cu: group level, group-major
+  uint32_t *gout;        // nxcd: release word per XCD
+  uint32_t *din;         // nxcd * sub_max: sub-group leaders' arrivals (device level)
+  uint32_t *sin;         // nxcd * sub_max * kSubArrive: members' arrivals (sub level)
+  uint32_t *lin, *lout;  // nab * per_cu: group level (local exchange), group-major
   uint32_t *episodes;
   int nab, per_cu, nxcd;
+  int sub_max;           // sub-groups per XCD (at most), as for XcdTree
 };
 
 // Whole-WG call.  el = group-level epoch, eg = episode (device level).
-// Arrival is gathered by flags level by level (group master <- members,
-// XCD leader <- group masters, device master <- XCD leaders).  The release
-// of a global episode is single-hop: the device master writes one release
-// word per XCD (gout[x]) and every WG of that XCD polls its word.  Only the
-// local-exchange barrier releases through the group's out-flags.  Flags
-// and data are agent-scope accesses, so no level needs a cache fence.
+// The global barrier gathers arrival flags on two levels, as the tree
+// barrier counts them: the WGs of one XCD form sub-groups of up to
+// kSubArrive (numbered as in xcd_tree_arrive), each leader gathers its
+// members' flags one per lane, and the device master (leader of sub-group 0)
+// gathers the leaders' flags one per lane.  The release is single-hop: the
+// device master writes one release word per XCD (gout[x]) and every WG of
+// that XCD polls its word.  The local-exchange barrier gathers the group's
+// in-flags and releases through its out-flags.  Flags and data are
+// agent-scope accesses, so no level needs a cache fence.
 __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int q,
                                                  uint32_t el, uint32_t eg,
                                                  bool local_only, uint32_t poll_q,
                                                  Guard &g, int *s_flag) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   constexpr long S = kFlagStride;
-  const int x = grp % f.nxcd;
-  const uint32_t *rel = f.gout + x * S;
   wait_mem();
   __syncthreads();
-  if (f.per_cu > 1) {
+  if (local_only) {
+    if (f.per_cu <= 1) return true;
     uint32_t *lin = f.lin + (long)grp * f.per_cu * S, *lout = f.lout + (long)grp * f.per_cu * S;
     if (q != 0) {
       if (tid == 0) {
         st_rlx(lin + q * S, el);
-        *s_flag = local_only ? poll_eq_pipe(lout + q * S, el, poll_q, g)
-                             : poll_eq_pipe(rel, eg, poll_q, g);
+        *s_flag = poll_eq_pipe(lout + q * S, el, poll_q, g);
       }
       __syncthreads();
       return *s_flag != 0;
@@ -657,44 +660,51 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
     bool ok = true;
     for (int m = 1 + tid; m < f.per_cu; m += nthr) ok = ok && poll_eq(lin + m * S, el, poll_q, g);
     if (!__syncthreads_and(ok)) return false;
-    if (local_only) {
-      for (int m = 1 + tid; m < f.per_cu; m += nthr) st_rlx(lout + m * S, el);
-      return true;
-    }
-  } else if (local_only) {
+    for (int m = 1 + tid; m < f.per_cu; m += nthr) st_rlx(lout + m * S, el);
     return true;
   }
-  const int k = grp / f.nxcd;
-  if (k != 0) {  // group master: arrive at the XCD level, wait for the release
+  const int x = grp % f.nxcd;
+  const uint32_t *rel = f.gout + x * S;
+  const int per = f.per_cu;
+  const int nwx = groups_on_xcd(f.nab, f.nxcd, x) * per;  // WGs on this XCD
+  const int wx = (grp / f.nxcd) * per + q;
+  const int k = wx / kSubArrive, m = wx % kSubArrive;
+  const int ck = nwx - k * kSubArrive < kSubArrive ? nwx - k * kSubArrive : kSubArrive;
+  const long sid = (long)x * f.sub_max + k;
+  uint32_t *sin = f.sin + sid * kSubArrive * S;
+  if (m != 0) {  // member: arrive at the sub level, wait for the release
     if (tid == 0) {
-      st_rlx(f.xin + grp * S, eg);
+      st_rlx(sin + m * S, eg);
       *s_flag = poll_eq_pipe(rel, eg, poll_q, g);
     }
     __syncthreads();
     return *s_flag != 0;
   }
-  // XCD leader: gather the XCD's group masters, one flag per lane
-  const int ngx = groups_on_xcd(f.nab, f.nxcd, x);
+  // sub-group leader: gather the members, one flag per lane
   bool ok = true;
-  for (int j = 1 + tid; j < ngx; j += nthr)
-    ok = ok && poll_eq(f.xin + (x + (long)j * f.nxcd) * S, eg, poll_q, g);
+  for (int j = 1 + tid; j < ck; j += nthr) ok = ok && poll_eq(sin + j * S, eg, poll_q, g);
+  if (!__syncthreads_and(ok)) return false;
+  if (sid != 0) {  // arrive at the device level, wait for the release
+    if (tid == 0) {
+      st_rlx(f.din + sid * S, eg);
+      *s_flag = poll_eq_pipe(rel, eg, poll_q, g);
+    }
+    __syncthreads();
+    return *s_flag != 0;
+  }
+  // device master: gather every other sub-group leader, one flag per lane,
+  // then release every XCD
+  const int nsub = f.nxcd * f.sub_max;
+  ok = true;
+  for (int y = 1 + tid; y < nsub; y += nthr) {
+    const int xy = y / f.sub_max, ky = y % f.sub_max;
+    if (ky * kSubArrive < groups_on_xcd(f.nab, f.nxcd, xy) * per)  // sub-group exists
+      ok = ok && poll_eq(f.din + (long)y * S, eg, poll_q, g);
+  }
   if (!__syncthreads_and(ok)) return false;
   const int nx = f.nab < f.nxcd ? f.nab : f.nxcd;
-  if (x != 0) {  // XCD leader: arrive at the device level, wait for the release
-    if (tid == 0) {
-      st_rlx(f.gin + x * S, eg);
-      *s_flag = poll_eq_pipe(rel, eg, poll_q, g);
-    }
-    __syncthreads();
-    return *s_flag != 0;
-  }
-  // device master: gather the other XCD leaders, one flag per lane, then
-  // release every XCD
-  ok = true;
-  for (int y = 1 + tid; y < nx; y += nthr) ok = ok && poll_eq(f.gin + y * S, eg, poll_q, g);
-  if (!__syncthreads_and(ok)) return false;
   if (tid == 0) st_rlx(f.episodes, ld_rlx(f.episodes) + 1);
-  for (int y = tid; y < nx; y += nthr) st_rlx(f.gout + y * S, eg);
+  for (int y = tid; y < nx; y += nthr) st_rlx(f.gout + (long)y * S, eg);
   return true;
 }
 
