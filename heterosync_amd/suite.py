@@ -131,7 +131,7 @@ def c4(dev, oh, iters=10000):
     return out
 
 
-def c5(dev, oh, n=256, seed=9, load=0.8, slots=16, depth=(6, 11)):
+def c5(dev, oh, n=256, seed=9, load=0.8, slots=16, depth=(6, 11), more_loads=(0.5, 0.95)):
     """UTS serving vs one launch per tree, geometric trees (b0 = 4) with
     depth limits drawn from `depth` (about 4K to 5M nodes each).
 
@@ -141,7 +141,8 @@ def c5(dev, oh, n=256, seed=9, load=0.8, slots=16, depth=(6, 11)):
     poisson: arrivals at `load` x the plain kernel capacity; latency =
       completion - arrival.  Plain latencies come from a FIFO queue served by
       the measured per-call wall times (and, as the plain side's best case,
-      by the kernel times alone).
+      by the kernel times alone).  `more_loads` repeats it at other offered
+      loads (poisson_by_load).
     parity: every tree's (nodes, leaves, max depth) is the same served in the
       mix as launched alone."""
     rng = np.random.default_rng(seed)
@@ -188,29 +189,37 @@ def c5(dev, oh, n=256, seed=9, load=0.8, slots=16, depth=(6, 11)):
            "speedup_vs_kernel": (summ.total_nodes / summ.wall_s) / capacity,
            "speedup_vs_wall": (summ.total_nodes / summ.wall_s) / (total_nodes / plain_wall)}
     ok = counts_ok(resp)
-    # Poisson arrivals
-    lam = load * capacity / (total_nodes / n)
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
-    arrivals -= arrivals[0]
-    resp, summ = dev.uts_serve([(sd, d, 4.0, float(a)) for (sd, d), a in zip(trees, arrivals)],
-                               slots=slots, timeout_s=120.0)
-    ok = ok and counts_ok(resp)
-    def fifo(col):
-        t, lat = 0.0, []
-        for a, p in zip(arrivals, plain):
-            t = max(t, a) + p[col]
-            lat.append(t - a)
-        lat = np.sort(lat)
-        return {"p50_s": float(lat[int(0.5 * (n - 1) + 0.5)]),
-                "p99_s": float(lat[int(0.99 * (n - 1) + 0.5)])}
 
-    pois = {"offered_load": load, "arrival_rate_per_s": lam,
+    def poisson(rho):
+        lam = rho * capacity / (total_nodes / n)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
+        arrivals -= arrivals[0]
+        resp, summ = dev.uts_serve(
+            [(sd, d, 4.0, float(a)) for (sd, d), a in zip(trees, arrivals)], slots=slots,
+            timeout_s=120.0)
+
+        def fifo(col):
+            t, lat = 0.0, []
+            for a, p in zip(arrivals, plain):
+                t = max(t, a) + p[col]
+                lat.append(t - a)
+            return {"p50_s": pct(lat, 0.5), "p99_s": pct(lat, 0.99)}
+
+        return counts_ok(resp), {
+            "offered_load": rho, "arrival_rate_per_s": lam,
             "serving": {"p50_s": summ.p50_s, "p99_s": summ.p99_s,
                         "nodes_per_s": summ.total_nodes / summ.wall_s,
                         **split(resp, arrivals)},
             "plain_fifo_kernel": fifo(3), "plain_fifo_wall": fifo(4)}
+
+    good, pois = poisson(load)
+    ok = ok and good
+    by_load = {}
+    for rho in more_loads:
+        good, by_load[str(rho)] = poisson(rho)
+        ok = ok and good
     return {"trees": n, "depths": list(depth), "total_nodes": int(total_nodes), "slots": slots,
-            "saturation": sat, "poisson": pois, "parity_ok": ok}
+            "saturation": sat, "poisson": pois, "poisson_by_load": by_load, "parity_ok": ok}
 
 
 def main(argv):

@@ -42,13 +42,20 @@ def test_all_sync_prims_cli(dev, name, pid):
         assert BANNER[pid] in lines, (pid, lines[:6])
 
 
-def test_all_sync_prims_cli_copy_mode_small_grid():
-    """The reference's own sweep shape: few WGs, numLdSt=10, copy mode."""
-    for name in ("atomicTreeBarrUniq", "lfTreeBarrUniqLocalExch", "spinMutexEBO", "faMutex",
-                 "sleepMutex", "spinSemEBO10", "spinSemUniq10", "atomicTreeBarrUniqSRB"):
-        for nw in (2, 4, 8, 16):
-            p = run("allSyncPrims-1kernel", name, 10, nw, 10)
-            assert p.returncode == 0 and "PASSED!" in p.stdout, (name, nw, p.stderr[-2000:])
+def test_results_sweep_reference_sets():
+    """syncPrims_results.sh's own sweep (its 16 primitives, numWGs 2 4 8 16,
+    numLdSt 10, numCSIters 10) through the sweep harness: every point passes
+    its self-check and yields the duration the script parses."""
+    import io
+    from heterosync_amd import sweep
+    out = io.StringIO()
+    rows = sweep.sweep(out=out)
+    assert len(rows) == len(sweep.SYNCPRIMS) * len(sweep.NUM_WGS)
+    bad = [r for r in rows if not r["passed"] or not r["average_ms"] > 0]
+    assert not bad, (bad[:3], out.getvalue()[-2000:])
+    for nw in (2, 4, 8, 16):  # and a sense-reversing barrier at the same shapes
+        p = run("allSyncPrims-1kernel", "atomicTreeBarrUniqSRB", 10, nw, 10)
+        assert p.returncode == 0 and "PASSED!" in p.stdout, (nw, p.stderr[-2000:])
 
 
 def test_all_sync_prims_cli_rejects_reserved_id():
