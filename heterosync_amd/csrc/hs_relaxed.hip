@@ -37,6 +37,14 @@ __device__ __forceinline__ void flush_count(uint64_t *dst, uint32_t *s_cnt) {
 //   variant 0: acquire/release scoped atomics, readers load (HQA:23-109)
 //   variant 1: relaxed atomics + agent fences, readers exchange 0 into the
 //              data as the reference's DeNovo-style "exch as load" (HQ:23-95)
+// STRICT keeps the reference's acquire/release orderings (each an agent-
+// scope cache write-back or invalidate, serialised per XCD).  Without it the
+// orderings come from completion alone: every access to the sequence word
+// and the rows is an agent-scope atomic (L1 bypassed, stores written
+// through), so a load performed after s_waitcnt has drained the wave's
+// earlier accesses sees them in order, and a store issued after the drain
+// becomes visible after them.  Readers: sequence load, row loads, drain,
+// sequence load.  Writers: CAS odd, row stores, drain, sequence store.
 struct SeqArgs {
   uint32_t *seq;    // n_seq words, each on its own line (stride 32)
   int32_t *data;    // n_seq * 2 * 64 words
@@ -47,7 +55,7 @@ struct SeqArgs {
   uint64_t ticks;
 };
 
-template <int VARIANT>
+template <int VARIANT, bool STRICT>
 __global__ void __launch_bounds__(1024) k_seqlock(SeqArgs a) {
   __shared__ uint32_t s_seq0, s_torn, s_ops;
   __shared__ int s_state;
@@ -70,15 +78,16 @@ __global__ void __launch_bounds__(1024) k_seqlock(SeqArgs a) {
   for (int it = 0; it < a.iters && alive; ++it) {
     if (writer) {
       if (is_master()) {  // HQ:109-130: wait for an even sequence, CAS it odd
-        uint32_t s0 = VARIANT == 0 ? ld_acq(seq) : ld_rlx(seq);
+        uint32_t s0 = (STRICT && VARIANT == 0) ? ld_acq(seq) : ld_rlx(seq);
         uint32_t ops = 1, bo = 1;
         int st = 1;
-        while ((s0 & 1u) || cas_acq(seq, s0, s0 + 1) != s0) {
+        while ((s0 & 1u) ||
+               (STRICT ? cas_acq(seq, s0, s0 + 1) : cas_rlx(seq, s0, s0 + 1)) != s0) {
           ++ops;
           if (!g.ok()) { st = -1; break; }
           sleep_quanta(bo);
           bo = ((bo << 1) + 1) & 127u;  // MAX_BACKOFF 128 (HQ:16)
-          s0 = VARIANT == 0 ? ld_acq(seq) : ld_rlx(seq);
+          s0 = (STRICT && VARIANT == 0) ? ld_acq(seq) : ld_rlx(seq);
           ++ops;
         }
         s_seq0 = s0;
@@ -89,16 +98,23 @@ __global__ void __launch_bounds__(1024) k_seqlock(SeqArgs a) {
       const uint32_t s0 = s_seq0;
       alive = s_state > 0;
       if (alive && wave0) {  // the 64-word rows belong to the first wave
-        if (VARIANT == 0) fence_rel(); else fence_acq();
+        if (STRICT) {
+          if (VARIANT == 0) fence_rel(); else fence_acq();
+        }
         __hip_atomic_store(d0 + lane, (int32_t)s0, __ATOMIC_RELAXED, HS_AGENT);
         __hip_atomic_store(d1 + lane, (int32_t)s0, __ATOMIC_RELAXED, HS_AGENT);
+        if (!STRICT) wait_mem();  // the rows are visible before the sequence moves on
       }
       __syncthreads();
-      if (alive && is_master()) st_rel(seq, s0 + 2);  // HQ:133-135
+      if (alive && is_master()) {  // HQ:133-135
+        if (STRICT) st_rel(seq, s0 + 2); else st_rlx(seq, s0 + 2);
+      }
     } else {
       for (;;) {  // HQ:23-56: retry until the sequence is even and unchanged
-        if (is_master()) {
-          s_seq0 = VARIANT == 0 ? ld_acq(seq) : (fence_acq(), ld_rlx(seq));
+        if (is_master()) {  // the value is stored, so the load has completed
+          s_seq0 = !STRICT         ? ld_rlx(seq)
+                   : VARIANT == 0 ? ld_acq(seq)
+                                  : (fence_acq(), ld_rlx(seq));
           s_ops += 2;
         }
         __syncthreads();
@@ -114,7 +130,8 @@ __global__ void __launch_bounds__(1024) k_seqlock(SeqArgs a) {
           }
         }
         if (is_master()) {
-          fence_acq();  // the row loads complete before the second read
+          // the row loads complete before the second read
+          if (STRICT) fence_acq(); else wait_mem();
           const uint32_t sb = ld_rlx(seq);
           s_state = (sa == sb && !(sa & 1u)) ? 1 : (g.ok() ? 0 : -1);
         }
@@ -350,11 +367,14 @@ extern "C" int hs_seqlock_run(hs_ctx *c, int num_wgs, int wg, int group_size,
                               uint64_t *torn, hs_relaxed_result *res) {
   if (!c || !seq_out || !out_arr || !res || num_wgs < 1 || wg < 64 || wg % 64 ||
       wg > 1024 || group_size < 1 || writer_period < 1 || iters < 0 ||
-      n_seq < group_size || (variant != 0 && variant != 1))
+      n_seq < group_size || variant < 0 || variant > 3)
     return HS_EINVAL;
   std::memset(res, 0, sizeof *res);
   HS_HIP(hipSetDevice(c->device));
-  void (*kfn)(SeqArgs) = variant == 0 ? k_seqlock<0> : k_seqlock<1>;
+  // bit 0: the reference variant; bit 1: its strict (fenced) orderings
+  static void (*const kKernels[4])(SeqArgs) = {k_seqlock<0, false>, k_seqlock<1, false>,
+                                               k_seqlock<0, true>, k_seqlock<1, true>};
+  void (*kfn)(SeqArgs) = kKernels[variant];
   int fit = 0, rc;
   // writers and readers of one lock wait on each other
   if ((rc = coresident_wgs(c, (const void *)kfn, wg, &fit))) return rc;

@@ -96,15 +96,17 @@ def c4(dev, oh, iters=10000):
     ncu = dev.info.num_cu
     nw = 4 * ncu
     out = {}
-    # seqlock: 1 writer WG per 64 readers (writer period 65), 16 locks
-    seq, outa, torn, r = dev.seqlock(nw, 256, 16, 65, iters, 0)
-    want = np.zeros(len(seq), np.uint32)
-    o.hso_seqlock_expected(nw, 16, 65, iters, want.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
-                           len(seq))
-    out["seqlock"] = {"ms": r.kernel_ms, "atomic_ops": r.atomic_ops,
-                      "atomic_ops_per_s": r.atomic_ops / (r.kernel_ms / 1e3),
-                      "passes_per_s": nw * iters / (r.kernel_ms / 1e3),
-                      "parity_ok": bool(np.array_equal(seq, want)) and torn == 0}
+    # seqlock: 1 writer WG per 64 readers (writer period 65), 16 locks; the
+    # default (drained agent-scope accesses) and the strict (fenced) form
+    for key, variant in (("seqlock", 0), ("seqlock_strict", 2)):
+        seq, outa, torn, r = dev.seqlock(nw, 256, 16, 65, iters, variant)
+        want = np.zeros(len(seq), np.uint32)
+        o.hso_seqlock_expected(nw, 16, 65, iters,
+                               want.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(seq))
+        out[key] = {"ms": r.kernel_ms, "atomic_ops": r.atomic_ops,
+                    "atomic_ops_per_s": r.atomic_ops / (r.kernel_ms / 1e3),
+                    "passes_per_s": nw * iters / (r.kernel_ms / 1e3),
+                    "parity_ok": bool(np.array_equal(seq, want)) and torn == 0}
     # flags
     rnd = np.zeros(nw * iters, np.float32)
     o.hso_flags_rand(rnd.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), rnd.size, 2018)

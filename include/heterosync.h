@@ -128,9 +128,12 @@ typedef struct {
 } hs_relaxed_result;
 
 /* seqlock: writer WGs are b % writer_period == 0 (reference: 16), lock
- * index b % group_size, `iters` passes per WG (reference: 10).  variant 0 =
- * scoped acquire/release atomics (seqlocks_hipAtomics), 1 = relaxed atomics
- * plus fences with exchanges as loads (seqlocks, useTFs).  seq_out has
+ * index b % group_size, `iters` passes per WG (reference: 10).  variant bit
+ * 0: 0 = scoped acquire/release atomics (seqlocks_hipAtomics), 1 = relaxed
+ * atomics plus fences with exchanges as loads (seqlocks, useTFs).  Variant
+ * bit 1 (variants 2, 3) keeps the reference's fences and acquire/release
+ * orderings; without it the same orderings come from drained agent-scope
+ * accesses with no cache fences (the default form).  seq_out has
  * n_seq >= group_size words; out_arr has num_wgs*wg_size words;
  * torn_reads counts reader snapshots whose two data words differed
  * (variant 0 only; must be 0). */

@@ -66,6 +66,7 @@ def test_all_sync_prims_cli_rejects_reserved_id():
 @pytest.mark.parametrize("args,ok_line,stream", [
     (("seqlocks", 512, 128, 4, 0), "PASSED", "stdout"),
     (("seqlocks", 512, 128, 4, 1), "PASSED", "stdout"),
+    (("seqlocks", 512, 128, 4, 0, "--strict"), "PASSED", "stdout"),
     (("flags", 160, 64, 16, 4, 0.33, 1, 1), "Test PASSED!", "stdout"),
     (("refCounter", 32, 64, 64, 4), "TEST PASSED!", "stderr"),
     (("splitCounter", 112, 128), "PASSED", "stdout"),  # writer and reader waves

@@ -18,7 +18,8 @@ def seq_expected(oracle, nw, gs, period, iters, n):
     return out
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+# bit 0: the reference variant (0 hipAtomics, 1 useTFs); bit 1: strict fences
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("nw,wg,gs,period,iters", [
     (512, 128, 4, 16, 10),     # the reference's run file: ./seqlocks 512 128 4 {0,1}
     (256, 64, 16, 65, 200),
@@ -29,7 +30,7 @@ def test_seqlock(dev, oracle, variant, nw, wg, gs, period, iters):
     want = seq_expected(oracle, nw, gs, period, iters, len(seq))
     assert np.array_equal(seq[:gs], want[:gs])
     assert all(int(x) % 2 == 0 for x in seq)
-    if variant == 0:
+    if variant % 2 == 0:  # readers load (do not exchange): snapshots are checkable
         assert torn == 0
         # every reader snapshot is r1 + r2 with r1 == r2 == a writer's even value
         readers = np.array([b for b in range(nw) if b % period], np.int64)
