@@ -141,7 +141,7 @@ def _lib():
         lib.hs_flags_run.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4 + [
             ctypes.c_float, ctypes.c_int, P(ctypes.c_float), P(ctypes.c_uint32),
             P(ctypes.c_uint32), P(RelaxedResult)]
-        lib.hs_refcounter_run.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [
+        lib.hs_refcounter_run.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [
             P(ctypes.c_uint32)] * 4 + [P(RelaxedResult)]
         lib.hs_splitcounter_run.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 3 + [
             P(ctypes.c_uint32), P(ctypes.c_uint32), P(RelaxedResult)]
@@ -262,7 +262,7 @@ class Device:
                                    ctypes.byref(res)), "hs_flags_run")
         return out, dirty.value, res
 
-    def refcounter(self, sharers, num_tbs, tb_size, num_repeats, spacing=-1):
+    def refcounter(self, sharers, num_tbs, tb_size, num_repeats, spacing=-1, strict=False):
         nthr = num_tbs * tb_size
         c0 = np.zeros(nthr // sharers, np.uint32)
         c1 = np.zeros_like(c0)
@@ -270,7 +270,7 @@ class Device:
         d1 = np.zeros_like(d0)
         res = RelaxedResult()
         _check(_lib().hs_refcounter_run(self._h, sharers, num_tbs, tb_size, num_repeats,
-                                        spacing, _ptr(c0, ctypes.c_uint32),
+                                        spacing, int(strict), _ptr(c0, ctypes.c_uint32),
                                         _ptr(c1, ctypes.c_uint32), _ptr(d0, ctypes.c_uint32),
                                         _ptr(d1, ctypes.c_uint32), ctypes.byref(res)),
                "hs_refcounter_run")

@@ -2,7 +2,8 @@
 // <tbSize> <numRepeats>` (cuda/relaxedAtomics/uvm/refCounter/refCounter.cu).
 // Checks refCounter.cu:85-126 (all counters 0, some del set) and the
 // stronger per-counter rule: every counter had a final decrement, so one of
-// its sharers marked del.  Option: --spacing=N (default 2*numRepeats).
+// its sharers marked del.  Options: --spacing=N (default 2*numRepeats),
+// --strict (the reference's fences around the decrements).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -12,14 +13,17 @@
 
 int main(int argc, char **argv) {
   if (argc < 5) {
-    std::fprintf(stderr, "./refCounter <numSharersPerGroup> <numTBs> <tbSize> <numRepeats> [--spacing=N]\n");
+    std::fprintf(stderr, "./refCounter <numSharersPerGroup> <numTBs> <tbSize> <numRepeats> "
+                         "[--spacing=N] [--strict]\n");
     return -1;
   }
   const int sh = std::atoi(argv[1]), ntb = std::atoi(argv[2]), tb = std::atoi(argv[3]);
   const int reps = std::atoi(argv[4]);
-  int spacing = -1;
-  for (int i = 5; i < argc; ++i)
+  int spacing = -1, strict = 0;
+  for (int i = 5; i < argc; ++i) {
     if (!std::strncmp(argv[i], "--spacing=", 10)) spacing = std::atoi(argv[i] + 10);
+    else if (!std::strcmp(argv[i], "--strict")) strict = 1;
+  }
   hs_ctx *ctx;
   if (hs_open(0, &ctx)) { std::fprintf(stderr, "no HIP device\n"); return -1; }
   const size_t nthr = (size_t)ntb * tb, nc = nthr / sh;
@@ -28,8 +32,8 @@ int main(int argc, char **argv) {
               nthr, ntb, nc, sh, groups, tb);
   std::vector<uint32_t> c0(nc), c1(nc), d0(nthr), d1(nthr);
   hs_relaxed_result r;
-  int rc = hs_refcounter_run(ctx, sh, ntb, tb, reps, spacing, c0.data(), c1.data(), d0.data(),
-                             d1.data(), &r);
+  int rc = hs_refcounter_run(ctx, sh, ntb, tb, reps, spacing, strict, c0.data(), c1.data(),
+                             d0.data(), d1.data(), &r);
   if (rc) { std::fprintf(stderr, "ERROR hs_refcounter_run - %s\n", hs_strerror(rc)); return -1; }
   bool pass = true;
   for (size_t i = 0; i < nc; ++i)

@@ -166,8 +166,15 @@ __global__ void __launch_bounds__(1024) k_seqlock(SeqArgs a) {
 // join the barrier.  Only it < numIters iterations can set dirty, so the
 // final count is exact: sum over workers and it < numIters of
 // [rand[b + it*numTBs] <= cutoff].
+// The reference's `dirty` is one word that every dirty iteration bumps
+// (CFK:138) and the main TB reads after the barrier (CFK:73); on MI355X RMWs to one address serialise at about 11 ns, so
+// that word alone bounds the run.  Here every dirty iteration still makes
+// one relaxed atomic add, to its own WG's line of `dirty`, and the main WG
+// sums the lines after the barrier (one per lane) before it tests the flag.
 struct FlagsArgs {
-  uint32_t *stop, *dirty;  // own lines
+  uint32_t *stop;   // own line
+  uint32_t *dirty;  // one line per WG
+  uint32_t *dirty_total;  // the main WG's sum, for the host
   uint32_t *out;
   const float *rnd;
   uint32_t *ctl;
@@ -180,8 +187,8 @@ struct FlagsArgs {
 
 __global__ void __launch_bounds__(1024) k_flags(FlagsArgs a) {
   __shared__ uint32_t lds[1024];
-  __shared__ int s_stop, s_ok, s_flag, s_dirty;
-  __shared__ uint32_t s_ops;
+  __shared__ int s_stop, s_ok, s_flag;
+  __shared__ uint32_t s_ops, s_dirty;
   Guard g;
   g.init(a.ctl, a.ticks);
   const int b = blockIdx.x, grid = gridDim.x;
@@ -218,7 +225,7 @@ __global__ void __launch_bounds__(1024) k_flags(FlagsArgs a) {
         for (int i = 0; i < dummy; ++i) lds[threadIdx.x]++;
         __syncthreads();
         if (is_master()) {
-          add_rlx(a.dirty, 1);
+          add_rlx(a.dirty + (long)b * HS_LINE_WORDS, 1);
           ++s_ops;
         }
       } else if (is_master()) {
@@ -238,9 +245,18 @@ __global__ void __launch_bounds__(1024) k_flags(FlagsArgs a) {
   if (main_wg && ok) {  // CFK:62-77
     if (is_master()) {
       fence_acq();
-      s_dirty = ld_rlx(a.dirty) != 0;
+      s_dirty = 0;
     }
     __syncthreads();
+    uint32_t part = 0;
+    for (int k = threadIdx.x; k < grid; k += blockDim.x)
+      part += ld_rlx(a.dirty + (long)k * HS_LINE_WORDS);
+    if (part) atomicAdd(&s_dirty, part);
+    __syncthreads();
+    if (is_master()) {
+      st_rlx(a.dirty_total, s_dirty);
+      s_ops += (uint32_t)grid;
+    }
     if (s_dirty) a.out[(long)b * blockDim.x + threadIdx.x] = lds[threadIdx.x];
   }
   __syncthreads();
@@ -261,6 +277,13 @@ struct RefArgs {
   int num_groups, repeats, spacing;
 };
 
+// STRICT keeps the agent-scope release fence before the decrements and the
+// acquire fence after a final one.  Without it the counters, the only
+// memory the benchmark shares, are agent-scope atomics: draining the wave's
+// increments (s_waitcnt) orders them before the decrements, and the del
+// store depends on the decrement's returned value, so no cache maintenance
+// (one write-back per wave per repeat, serialised per XCD) is needed.
+template <bool STRICT>
 __global__ void __launch_bounds__(1024) k_refcounter(RefArgs a) {
   __shared__ volatile int lds[1024];
   const int b = blockIdx.x, t = threadIdx.x;
@@ -276,10 +299,10 @@ __global__ void __launch_bounds__(1024) k_refcounter(RefArgs a) {
       lds[t] += j;
       __syncthreads();
     }
-    fence_rel();
+    if (STRICT) fence_rel(); else wait_mem();
     const uint32_t o0 = __builtin_amdgcn_atomic_dec32(p0, 1000000000u, __ATOMIC_RELAXED, "agent");
     const uint32_t o1 = __builtin_amdgcn_atomic_dec32(p1, 1000000000u, __ATOMIC_RELAXED, "agent");
-    if (o0 <= 1 || o1 <= 1) fence_acq();
+    if (STRICT && (o0 <= 1 || o1 <= 1)) fence_acq();
     if (o0 <= 1) a.del0[me] = 1;
     if (o1 <= 1) a.del1[me] = 1;
   }
@@ -436,13 +459,13 @@ extern "C" int hs_flags_run(hs_ctx *c, int num_tbs, int tb, int accs, int repeat
   uint64_t *d_stats;
   const size_t nrand = (size_t)num_tbs * iters;
   const size_t flag_words = 2 * (size_t)nab + 2 * (size_t)nab * per + HS_LINE_WORDS;
-  if ((rc = ar.alloc(&d_sync, 4 * HS_LINE_WORDS))) return rc;
+  if ((rc = ar.alloc(&d_sync, (4 + (size_t)num_tbs) * HS_LINE_WORDS))) return rc;
   if ((rc = ar.alloc(&d_out, (size_t)num_tbs * tb))) return rc;
   if ((rc = ar.alloc(&d_ctl, 64))) return rc;
   if ((rc = ar.alloc(&d_flags, flag_words))) return rc;
   if ((rc = ar.alloc(&d_rnd, nrand))) return rc;
   if ((rc = ar.alloc(&d_stats, 4))) return rc;
-  HS_HIP(hipMemset(d_sync, 0, 4 * HS_LINE_WORDS * 4));
+  HS_HIP(hipMemset(d_sync, 0, (4 + (size_t)num_tbs) * HS_LINE_WORDS * 4));
   HS_HIP(hipMemset(d_out, 0, (size_t)num_tbs * tb * 4));
   HS_HIP(hipMemset(d_ctl, 0, 256));
   HS_HIP(hipMemset(d_flags, 0, flag_words * 4));
@@ -450,7 +473,8 @@ extern "C" int hs_flags_run(hs_ctx *c, int num_tbs, int tb, int accs, int repeat
   if (nrand) HS_HIP(hipMemcpy(d_rnd, rnd, nrand * 4, hipMemcpyHostToDevice));
   FlagsArgs a;
   a.stop = d_sync;
-  a.dirty = d_sync + 2 * HS_LINE_WORDS;
+  a.dirty_total = d_sync + 2 * HS_LINE_WORDS;
+  a.dirty = d_sync + 4 * HS_LINE_WORDS;
   a.out = d_out;
   a.rnd = d_rnd;
   a.ctl = d_ctl;
@@ -474,13 +498,13 @@ extern "C" int hs_flags_run(hs_ctx *c, int num_tbs, int tb, int accs, int repeat
   HS_HIP(hipEventElapsedTime(&ms, t.e0, t.e1));
   res->kernel_ms = ms;
   HS_HIP(hipMemcpy(out, d_out, (size_t)num_tbs * tb * 4, hipMemcpyDeviceToHost));
-  HS_HIP(hipMemcpy(dirty, a.dirty, 4, hipMemcpyDeviceToHost));
+  HS_HIP(hipMemcpy(dirty, a.dirty_total, 4, hipMemcpyDeviceToHost));
   return read_stats(d_stats, d_ctl, res, nullptr);
 }
 
 extern "C" int hs_refcounter_run(hs_ctx *c, int sharers, int num_tbs, int tb, int repeats,
-                                 int spacing, uint32_t *c0, uint32_t *c1, uint32_t *del0,
-                                 uint32_t *del1, hs_relaxed_result *res) {
+                                 int spacing, int strict, uint32_t *c0, uint32_t *c1,
+                                 uint32_t *del0, uint32_t *del1, hs_relaxed_result *res) {
   if (!c || !c0 || !c1 || !del0 || !del1 || !res || sharers < 1 || num_tbs < sharers ||
       num_tbs % sharers || tb < 1 || tb > 1024 || repeats < 0)
     return HS_EINVAL;
@@ -501,7 +525,10 @@ extern "C" int hs_refcounter_run(hs_ctx *c, int sharers, int num_tbs, int tb, in
   Timer t;
   HS_HIP(hipDeviceSynchronize());
   HS_HIP(hipEventRecord(t.e0, c->stream));
-  hipLaunchKernelGGL(k_refcounter, dim3(num_tbs), dim3(tb), 0, c->stream, a);
+  if (strict)
+    hipLaunchKernelGGL(k_refcounter<true>, dim3(num_tbs), dim3(tb), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_refcounter<false>, dim3(num_tbs), dim3(tb), 0, c->stream, a);
   HS_HIP(hipGetLastError());
   HS_HIP(hipEventRecord(t.e1, c->stream));
   HS_HIP(hipEventSynchronize(t.e1));

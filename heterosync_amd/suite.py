@@ -116,14 +116,17 @@ def c4(dev, oh, iters=10000):
     out["flags"] = {"ms": r.kernel_ms, "dirty": dirty,
                     "worker_iters_per_s": (nw - 1) * iters / (r.kernel_ms / 1e3),
                     "parity_ok": dirty == wd}
-    # refCounter: 32 sharers, 10^4 inc/dec pairs, spacing 8
-    c0, c1_, d0, d1, r = dev.refcounter(32, nw, 256, iters, 8)
-    groups = nw // 32
-    okr = not c0.any() and not c1_.any() and bool(
-        np.all(d0.reshape(32, groups, 256).max(axis=0) == 1))
-    out["refCounter"] = {"ms": r.kernel_ms, "atomic_ops": r.atomic_ops,
-                         "atomic_ops_per_s": r.atomic_ops / (r.kernel_ms / 1e3),
-                         "parity_ok": okr}
+    # refCounter: 32 sharers, 10^4 inc/dec pairs, spacing 8; default and
+    # strict (fenced) forms
+    for key, strict in (("refCounter", False), ("refCounter_strict", True)):
+        c0, c1_, d0, d1, r = dev.refcounter(32, nw, 256, iters, 8, strict=strict)
+        groups = nw // 32
+        okr = not c0.any() and not c1_.any() and bool(
+            np.all(d0.reshape(32, groups, 256).max(axis=0) == 1)) and bool(
+            np.all(d1.reshape(32, groups, 256).max(axis=0) == 1))
+        out[key] = {"ms": r.kernel_ms, "atomic_ops": r.atomic_ops,
+                    "atomic_ops_per_s": r.atomic_ops / (r.kernel_ms / 1e3),
+                    "parity_ok": okr}
     # split counter
     cnt, so, r = dev.splitcounter(nw, 256, iters)
     out["splitCounter"] = {"ms": r.kernel_ms, "atomic_ops": r.atomic_ops,

@@ -152,10 +152,12 @@ int hs_flags_run(hs_ctx *ctx, int num_tbs, int tb_size, int num_accs_per_thr,
 
 /* refCounter: num_repeats x {+1 on c0,c1; `spacing` LDS steps;
  * atomicDec(c, 1e9) on c0, c1; del = old <= 1}.  spacing < 0 = the
- * reference's 2*num_repeats.  c0/c1 have num_tbs*tb_size/sharers words,
+ * reference's 2*num_repeats.  strict = 1 keeps the fences around the
+ * decrements (refCounter_kernel.cu:42-45); 0 orders them by drained
+ * agent-scope atomics.  c0/c1 have num_tbs*tb_size/sharers words,
  * del0/del1 num_tbs*tb_size. */
 int hs_refcounter_run(hs_ctx *ctx, int sharers, int num_tbs, int tb_size,
-                      int num_repeats, int spacing, uint32_t *c0,
+                      int num_repeats, int spacing, int strict, uint32_t *c0,
                       uint32_t *c1, uint32_t *del0, uint32_t *del1,
                       hs_relaxed_result *res);
 

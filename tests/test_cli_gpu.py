@@ -69,6 +69,7 @@ def test_all_sync_prims_cli_rejects_reserved_id():
     (("seqlocks", 512, 128, 4, 0, "--strict"), "PASSED", "stdout"),
     (("flags", 160, 64, 16, 4, 0.33, 1, 1), "Test PASSED!", "stdout"),
     (("refCounter", 32, 64, 64, 4), "TEST PASSED!", "stderr"),
+    (("refCounter", 32, 64, 64, 4, "--strict"), "TEST PASSED!", "stderr"),
     (("splitCounter", 112, 128), "PASSED", "stdout"),  # writer and reader waves
     (("uts", 1, 8192, 90, 32, 128), "SUCCESS!", "stdout"),
 ])

@@ -62,13 +62,14 @@ def test_flags_never_dirty(dev):
     assert dirty == 0 and not out.any()
 
 
+@pytest.mark.parametrize("strict", [False, True])
 @pytest.mark.parametrize("sh,ntb,tb,reps,spacing", [
     (32, 64, 32, 4, -1),            # the reference's run file
     (32, 1024, 256, 100, 8),
     (4, 512, 64, 50, 0),
 ])
-def test_refcounter(dev, sh, ntb, tb, reps, spacing):
-    c0, c1, d0, d1, r = dev.refcounter(sh, ntb, tb, reps, spacing)
+def test_refcounter(dev, sh, ntb, tb, reps, spacing, strict):
+    c0, c1, d0, d1, r = dev.refcounter(sh, ntb, tb, reps, spacing, strict=strict)
     assert not c0.any() and not c1.any()
     groups = ntb // sh
     for d in (d0, d1):
