@@ -268,7 +268,9 @@ __device__ void dispatch(const UtsArgs &a, Guard &g) {
 }
 
 // The work-stealing worker (one wave per WG); the kernels below wrap it.
-template <int TREE, bool SERVE>
+// STATS: per-WG timing and hand-off counters (a separate instantiation, so
+// the kernel that runs by default spends no registers on them).
+template <int TREE, bool SERVE, bool STATS>
 __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   constexpr int W = Rec<TREE>::W;
   __shared__ uint32_t s_pst[W * kWave];   // parent records of the batch
@@ -313,21 +315,21 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   // phase clock: adds the time since the last call to `acc`
   unsigned long long t_ph = 0;
   auto phase = [&](unsigned long long &acc) {
-    if (!a.stats) return;
+    if (!STATS) return;
     const unsigned long long now = wall_clock64();
     acc += now - t_ph;
     t_ph = now;
   };
-  unsigned long long t_mark = a.stats ? wall_clock64() : 0;
+  unsigned long long t_mark = STATS ? wall_clock64() : 0;
   // folds the time since the last mark into busy or idle
   auto tick = [&](bool busy) {
-    if (!a.stats) return;
+    if (!STATS) return;
     const unsigned long long now = wall_clock64();
     (busy ? st_busy : st_idle) += now - t_mark;
     t_mark = now;
   };
   auto flush_stats = [&]() {
-    if (!a.stats || lane != 0) return;
+    if (!STATS || lane != 0) return;
     unsigned long long *o = a.stats + (long)blockIdx.x * kStatWords;
     o[0] = st_busy;
     o[1] = st_idle;
@@ -493,7 +495,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   for (;;) {
     const long held = top - bot;
     if (held > 0) {
-      if (a.stats) t_ph = wall_clock64();
+      if (STATS) t_ph = wall_clock64();
       // a delivery for a request still outstanding, read now, used after
       const uint32_t box_n = (lane == 0 && requested) ? ld_rlx(my_n) : 0;
       // ---- pop up to 64 parents ------------------------------------------
@@ -753,12 +755,19 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
 template <int TREE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
 k_uts(UtsArgs a) {
-  uts_worker<TREE, false>(a);
+  uts_worker<TREE, false, false>(a);
+}
+
+// The same with per-WG statistics (HS_UTS_STATS=1).
+template <int TREE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+k_uts_stats(UtsArgs a) {
+  uts_worker<TREE, false, true>(a);
 }
 
 // The persistent serving loop runs two waves per SIMD (small trees gain
 // nothing from a bigger grid), so it keeps its full register allocation.
-__global__ void __launch_bounds__(64) k_uts_serve(UtsArgs a) { uts_worker<1, true>(a); }
+__global__ void __launch_bounds__(64) k_uts_serve(UtsArgs a) { uts_worker<1, true, false>(a); }
 
 __global__ void k_root_digest(uint32_t seed, uint32_t *out) {
   uint32_t st[5];
@@ -989,7 +998,10 @@ int run_single(hs_ctx *c, Engine &e, uint32_t seed, uint32_t gen_mx, hs_uts_resu
   HS_HIP(hipEventCreate(&e0));
   HS_HIP(hipEventCreate(&e1));
   HS_HIP(hipEventRecord(e0, c->stream));
-  hipLaunchKernelGGL((k_uts<TREE>), dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
+  if (e.a.stats)
+    hipLaunchKernelGGL((k_uts_stats<TREE>), dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
+  else
+    hipLaunchKernelGGL((k_uts<TREE>), dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
   HS_HIP(hipGetLastError());
   HS_HIP(hipEventRecord(e1, c->stream));
   HS_HIP(hipEventSynchronize(e1));
