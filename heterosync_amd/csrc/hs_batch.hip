@@ -266,38 +266,40 @@ int batch_launch(hs_ctx *c, Batch &B, int n, const hs_batch_cfg *cfg, int reps, 
   a.bo.min_q = cfg->backoff_min > 0 ? cfg->backoff_min : 1;
   a.bo.max_q = cfg->backoff_max > 0 ? cfg->backoff_max : 1023;
   a.bo.poll_q = cfg->poll_sleep < 0 ? 0 : (cfg->poll_sleep > 0 ? cfg->poll_sleep : 1);
-  hipEvent_t e0, e1;
-  HS_HIP(hipEventCreate(&e0));
-  HS_HIP(hipEventCreate(&e1));
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  EventSet ev;
+  int rc;
+  if ((rc = ev.create(2))) return rc;
+  struct Graph {  // destroyed on every return path
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    ~Graph() {
+      if (x) (void)hipGraphExecDestroy(x);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } gr;
   if (cfg->use_graph) {
     HS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     for (int r = 0; r < reps; ++r)
       hipLaunchKernelGGL(k_batch, dim3(P), dim3(64), 0, c->stream, a);
-    HS_HIP(hipStreamEndCapture(c->stream, &graph));
-    HS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HS_HIP(hipStreamEndCapture(c->stream, &gr.g));
+    HS_HIP(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0));
   }
   HS_HIP(hipDeviceSynchronize());
-  HS_HIP(hipEventRecord(e0, c->stream));
+  HS_HIP(hipEventRecord(ev[0], c->stream));
   if (cfg->use_graph) {
-    HS_HIP(hipGraphLaunch(exec, c->stream));
+    HS_HIP(hipGraphLaunch(gr.x, c->stream));
   } else {
     for (int r = 0; r < reps; ++r) {
       hipLaunchKernelGGL(k_batch, dim3(P), dim3(64), 0, c->stream, a);
       HS_HIP(hipGetLastError());
     }
   }
-  HS_HIP(hipEventRecord(e1, c->stream));
-  HS_HIP(hipEventSynchronize(e1));
+  HS_HIP(hipEventRecord(ev[1], c->stream));
+  HS_HIP(hipEventSynchronize(ev[1]));
   float ms = 0;
-  HS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  HS_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
   *roi_ms = ms;
   if (kern_ms) *kern_ms = ms;
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return HS_OK;
 }
 

@@ -58,6 +58,29 @@ class DevArena {
   std::vector<void *> ptrs_;
 };
 
+// Owns a set of HIP events; destroys them on every return path.
+class EventSet {
+ public:
+  ~EventSet() {
+    for (hipEvent_t e : ev_)
+      if (e) (void)hipEventDestroy(e);
+  }
+  int create(size_t n) {
+    ev_.assign(n, nullptr);
+    for (hipEvent_t &e : ev_)
+      if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+        (void)hipGetLastError();
+        return HS_EHIP;
+      }
+    return HS_OK;
+  }
+  hipEvent_t operator[](size_t i) const { return ev_[i]; }
+
+ private:
+  std::vector<hipEvent_t> ev_;
+};
+
 inline uint64_t timeout_ticks(const hs_ctx *c, double s) {
   if (s <= 0) s = 20.0;
   return (uint64_t)(s * c->wall_hz);

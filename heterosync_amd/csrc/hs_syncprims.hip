@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(1024) k_mutex(SPArgs a) {
     if (is_master()) s_ok = mutex_lock<KIND, AGENT>(lk, ring, a.ring_stride, ticket, a.bo, g);
     __syncthreads();
     if (!s_ok) return;
-    wg_move<MODE, AGENT>(a.storage, tid0, a.L);
+    wg_move<MODE, AGENT, 32>(a.storage, tid0, a.L);
     if (AGENT) wait_mem();
     __syncthreads();
     if (is_master()) {
@@ -96,12 +96,12 @@ __global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
       if (UNIQ) {  // main.hip.cpp:1040-1058: own region, then +numCU WGs
         int cur = b;
         for (int j = 0; j < per; ++j) {
-          wg_move<MODE, AGENT>(a.storage, (long)cur * blockDim.x, a.L);
+          wg_move<MODE, AGENT, 32>(a.storage, (long)cur * blockDim.x, a.L);
           cur = (cur + nab) % grid;
         }
       } else {  // main.hip.cpp:903-915: regions 0..perCU-1
         for (int j = 0; j < per; ++j)
-          wg_move<MODE, AGENT>(a.storage, (long)j * blockDim.x, a.L);
+          wg_move<MODE, AGENT, 32>(a.storage, (long)j * blockDim.x, a.L);
       }
     } else {
       wg_read<AGENT>(a.storage, (long)(UNIQ ? b : pw) * blockDim.x, a.L, sink);
@@ -559,11 +559,10 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   const uint32_t local_per_iter = (lx && per > 1) ? 2 : 1;
 
   // ---- region of interest (main.hip.cpp:2354-2511) -----------------------
-  hipEvent_t ev0, ev1;
-  HS_HIP(hipEventCreate(&ev0));
-  HS_HIP(hipEventCreate(&ev1));
-  std::vector<hipEvent_t> kev(2 * reps);
-  for (auto &e : kev) HS_HIP(hipEventCreate(&e));
+  EventSet ev;  // [0] ROI start, [1] ROI end, then a pair per launch
+  if ((rc = ev.create(2 + 2 * (size_t)reps))) return rc;
+  const hipEvent_t ev0 = ev[0], ev1 = ev[1];
+  auto kev = [&](int i) { return ev[2 + i]; };
   HS_HIP(hipDeviceSynchronize());
   HS_HIP(hipEventRecord(ev0, c->stream));
   int launch_rc = HS_OK;
@@ -571,10 +570,10 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
     a.xcd_map = d_xmap + (size_t)r * nxcd;
     a.e_global0 = (uint32_t)r * iters;
     a.e_local0 = (uint32_t)r * iters * local_per_iter;
-    HS_HIP(hipEventRecord(kev[2 * r], c->stream));
+    HS_HIP(hipEventRecord(kev(2 * r), c->stream));
     hipLaunchKernelGGL(kfn, dim3(nw), dim3(wg), 0, c->stream, a);
     if (hipGetLastError() != hipSuccess) { launch_rc = HS_EHIP; break; }
-    HS_HIP(hipEventRecord(kev[2 * r + 1], c->stream));
+    HS_HIP(hipEventRecord(kev(2 * r + 1), c->stream));
     HS_HIP(hipStreamSynchronize(c->stream));
   }
   HS_HIP(hipEventRecord(ev1, c->stream));
@@ -585,13 +584,10 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   double kms = 0;
   for (int r = 0; r < reps && launch_rc == HS_OK; ++r) {
     float k = 0;
-    HS_HIP(hipEventElapsedTime(&k, kev[2 * r], kev[2 * r + 1]));
+    HS_HIP(hipEventElapsedTime(&k, kev(2 * r), kev(2 * r + 1)));
     kms += k;
   }
   res->kernel_ms = kms;
-  for (auto &e : kev) (void)hipEventDestroy(e);
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
   if (launch_rc) return launch_rc;
 
   // ---- results and idle-state invariants ----------------------------------

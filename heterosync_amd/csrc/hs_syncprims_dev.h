@@ -60,13 +60,19 @@ __device__ __forceinline__ int32_t ld_data(const int32_t *p) {
   return *p;
 }
 
-// N source lines starting at `lo`: N independent loads, then N stores.
+// N source lines of a thread's column starting at `lo`: N independent
+// loads (MODE 0: lines lo..lo+N-1; MODE 1: lines lo+1..lo+N), and the
+// stores of lines lo+1..lo+N (copied, or incremented in place).  N is a
+// compile-time constant so the accesses are straight-line code that the
+// compiler keeps in flight together.
 template <int MODE, int N, bool SC1>
-__device__ __forceinline__ void shift_chunk_x(int32_t *__restrict__ col, int lo) {
-  int32_t v[N];
+__device__ __forceinline__ void chunk_ld(const int32_t *col, int lo, int32_t (&v)[16]) {
 #pragma unroll
   for (int k = 0; k < N; ++k)
     v[k] = ld_data<SC1>(col + (lo + k + (MODE == 0 ? 0 : 1)) * kWordsPerLine);
+}
+template <int MODE, int N, bool SC1>
+__device__ __forceinline__ void chunk_st(int32_t *col, int lo, const int32_t (&v)[16]) {
 #pragma unroll
   for (int k = N - 1; k >= 0; --k) {
     const int32_t val = MODE == 0 ? v[k] : v[k] + 1;
@@ -77,30 +83,56 @@ __device__ __forceinline__ void shift_chunk_x(int32_t *__restrict__ col, int lo)
   }
 }
 
+// The bottom chunk [0, r), r < 16 lines: loads or stores by a switch over
+// the compile-time sizes.
+template <int MODE, bool SC1, bool STORE>
+__device__ __forceinline__ void chunk_tail(int32_t *col, int r, int32_t (&v)[16]) {
+  switch (r) {
+#define HS_TAIL(n)                                          \
+  case n:                                                   \
+    if (STORE) chunk_st<MODE, n, SC1>(col, 0, v);           \
+    else chunk_ld<MODE, n, SC1>(col, 0, v);                 \
+    break;
+    HS_TAIL(1) HS_TAIL(2) HS_TAIL(3) HS_TAIL(4) HS_TAIL(5) HS_TAIL(6) HS_TAIL(7) HS_TAIL(8)
+    HS_TAIL(9) HS_TAIL(10) HS_TAIL(11) HS_TAIL(12) HS_TAIL(13) HS_TAIL(14) HS_TAIL(15)
+#undef HS_TAIL
+    default: break;
+  }
+}
+
+// The walk goes top-down: full chunks of 16 source lines, then the bottom
+// chunk of L % 16.  A chunk's loads are issued before the chunk above it is
+// stored: they read lines that chunk does not write (it stores lines
+// lo+1..lo+16, the chunk below reads lines under lo), and the lowest line a
+// chunk reads is overwritten only by the stores of the chunk below, issued
+// after that chunk's own loads have returned (vector memory operations
+// complete in order).  The memory counter counts loads and stores in issue
+// order, so waiting for a chunk's loads never waits for stores issued after
+// them: each chunk costs one load round trip instead of a load and a store.
 template <int MODE, bool SC1 = false>
 __device__ __forceinline__ void access_data(int32_t *__restrict__ s,
                                             long base_line, int off, int L) {
+  if (L <= 0) return;
   int32_t *col = s + base_line * kWordsPerLine + off;
-  int hi = L;  // source lines [0, hi) remain, processed from the top down
-  for (; hi >= 16; hi -= 16) shift_chunk_x<MODE, 16, SC1>(col, hi - 16);
-  switch (hi) {
-    case 15: shift_chunk_x<MODE, 15, SC1>(col, 0); break;
-    case 14: shift_chunk_x<MODE, 14, SC1>(col, 0); break;
-    case 13: shift_chunk_x<MODE, 13, SC1>(col, 0); break;
-    case 12: shift_chunk_x<MODE, 12, SC1>(col, 0); break;
-    case 11: shift_chunk_x<MODE, 11, SC1>(col, 0); break;
-    case 10: shift_chunk_x<MODE, 10, SC1>(col, 0); break;
-    case 9: shift_chunk_x<MODE, 9, SC1>(col, 0); break;
-    case 8: shift_chunk_x<MODE, 8, SC1>(col, 0); break;
-    case 7: shift_chunk_x<MODE, 7, SC1>(col, 0); break;
-    case 6: shift_chunk_x<MODE, 6, SC1>(col, 0); break;
-    case 5: shift_chunk_x<MODE, 5, SC1>(col, 0); break;
-    case 4: shift_chunk_x<MODE, 4, SC1>(col, 0); break;
-    case 3: shift_chunk_x<MODE, 3, SC1>(col, 0); break;
-    case 2: shift_chunk_x<MODE, 2, SC1>(col, 0); break;
-    case 1: shift_chunk_x<MODE, 1, SC1>(col, 0); break;
-    default: break;
+  const int full = L / 16, r = L % 16;
+  int32_t a[16], b[16];
+  if (full == 0) {
+    chunk_tail<MODE, SC1, false>(col, r, a);
+    chunk_tail<MODE, SC1, true>(col, r, a);
+    return;
   }
+  int lo = L - 16;
+  chunk_ld<MODE, 16, SC1>(col, lo, a);
+  for (int f = 1; f < full; ++f) {
+    chunk_ld<MODE, 16, SC1>(col, lo - 16, b);
+    chunk_st<MODE, 16, SC1>(col, lo, a);
+    lo -= 16;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = b[k];
+  }
+  if (r > 0) chunk_tail<MODE, SC1, false>(col, r, b);
+  chunk_st<MODE, 16, SC1>(col, lo, a);
+  if (r > 0) chunk_tail<MODE, SC1, true>(col, r, b);
 }
 
 // Lane `t` of a WG whose data region starts at global thread id `tid0`.
@@ -114,10 +146,12 @@ __device__ __forceinline__ void wg_access(int32_t *s, long tid0, int L) {
 // The same pass done by the whole WG with 16-byte accesses.  A WG's region
 // is the contiguous chains of its blockDim threads (blockDim is a multiple
 // of 64, so the region starts on a chain), and each chain is L+1 lines of
-// 64 B.  The WG moves it band by band, top band first (at most 16 source
-// lines, i.e. 4 pieces per thread), with all loads of a band complete before
-// any of its stores -- exactly the per-thread top-down walk's result, since
-// a band's stores only overwrite lines its own loads have already read.
+// 64 B.  The WG moves it band by band, top band first (at most BAND source
+// lines, i.e. BAND/4 pieces per thread), with all loads of a band complete
+// before any of its stores -- exactly the per-thread top-down walk's result,
+// since a band's stores only overwrite lines its own loads have already
+// read.  Each band costs a load and a store round trip, so a long walk
+// (numLdSt = 100) takes taller bands: BAND = 64 moves it in two.
 // SC1: agent-scope accesses (L1 bypassed, stores written through).
 // keep_l2 (with SC1): the region's next access is by a WG on this XCD, which
 // stores the region again.  Plain stores then leave the lines dirty in this
@@ -130,8 +164,10 @@ __device__ __forceinline__ void wg_access(int32_t *s, long tid0, int L) {
 typedef uint32_t hs_v4u __attribute__((ext_vector_type(4)));
 constexpr int kRsrcWord3 = 0x00020000;  // raw buffer descriptor, gfx9 family
 
-template <int MODE, bool SC1 = false>
+template <int MODE, bool SC1 = false, int BAND = 16>
 __device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L, bool keep_l2 = false) {
+  static_assert(BAND % 16 == 0 && BAND >= 16 && BAND <= 64, "band of 16..64 lines");
+  constexpr int kItems = BAND / 4;  // 16-byte pieces per thread per band
   if (L <= 0) return;
   const int nt = (int)blockDim.x, t = (int)threadIdx.x;
   const int nc = nt / kWordsPerLine;  // chains in the region
@@ -139,12 +175,12 @@ __device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L, bool keep_
   const int bytes = nc * (L + 1) * kWordsPerLine * 4;
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(base, 0, bytes, kRsrcWord3);
   constexpr int kAux = SC1 ? 16 : 0;  // cache policy: sc1
-  for (int hi = L; hi > 0; hi -= 16) {  // source lines [lo, hi)
-    const int lo = hi > 16 ? hi - 16 : 0, nb = hi - lo, items = nc * nb * 4;
-    hs_v4u v[4];
-    int dst[4];
+  for (int hi = L; hi > 0; hi -= BAND) {  // source lines [lo, hi)
+    const int lo = hi > BAND ? hi - BAND : 0, nb = hi - lo, items = nc * nb * 4;
+    hs_v4u v[kItems];
+    int dst[kItems];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kItems; ++k) {
       const int q = t + k * nt;
       dst[k] = -1;
       if (q < items) {
@@ -160,12 +196,12 @@ __device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L, bool keep_
     }
     if (SC1 && !keep_l2) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
+      for (int k = 0; k < kItems; ++k)
         if (dst[k] >= 0)
           __builtin_amdgcn_raw_buffer_store_b128(MODE == 0 ? v[k] : v[k] + 1u, rs, dst[k], 0, kAux);
     } else {
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
+      for (int k = 0; k < kItems; ++k)
         if (dst[k] >= 0)
           __builtin_amdgcn_raw_buffer_store_b128(MODE == 0 ? v[k] : v[k] + 1u, rs, dst[k], 0, 0);
     }

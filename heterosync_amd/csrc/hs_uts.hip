@@ -994,21 +994,18 @@ int run_single(hs_ctx *c, Engine &e, uint32_t seed, uint32_t gen_mx, hs_uts_resu
   if ((rc = stats_begin(e, c->stream))) return rc;
   hipLaunchKernelGGL(k_seed_root, dim3(1), dim3(kWave), 0, c->stream, e.a, TREE, seed, gen_mx);
   HS_HIP(hipGetLastError());
-  hipEvent_t e0, e1;
-  HS_HIP(hipEventCreate(&e0));
-  HS_HIP(hipEventCreate(&e1));
-  HS_HIP(hipEventRecord(e0, c->stream));
+  EventSet ev;
+  if ((rc = ev.create(2))) return rc;
+  HS_HIP(hipEventRecord(ev[0], c->stream));
   if (e.a.stats)
     hipLaunchKernelGGL((k_uts_stats<TREE>), dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
   else
     hipLaunchKernelGGL((k_uts<TREE>), dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
   HS_HIP(hipGetLastError());
-  HS_HIP(hipEventRecord(e1, c->stream));
-  HS_HIP(hipEventSynchronize(e1));
+  HS_HIP(hipEventRecord(ev[1], c->stream));
+  HS_HIP(hipEventSynchronize(ev[1]));
   float ms = 0;
-  HS_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  HS_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
   res->kernel_ms = ms;
   if ((rc = stats_report(e, ms))) return rc;
   return e.read_slot(c->stream, 0, res);
