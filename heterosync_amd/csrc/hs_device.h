@@ -48,9 +48,6 @@ __device__ __forceinline__ uint32_t add_rel(uint32_t *p, uint32_t v) {
 __device__ __forceinline__ uint32_t add_acqrel(uint32_t *p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, HS_AGENT);
 }
-__device__ __forceinline__ uint32_t exch_rlx(uint32_t *p, uint32_t v) {
-  return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, HS_AGENT);
-}
 // CAS returning the old value (CUDA atomicCAS convention).
 __device__ __forceinline__ uint32_t cas_acq(uint32_t *p, uint32_t expected,
                                             uint32_t desired) {

@@ -91,10 +91,21 @@ def test_uniq_semaphore(dev, ohelp, prim, scope):
 @pytest.mark.parametrize("scope", [0, 1])
 def test_barrier_small(dev, ohelp, prim, mode, scope):
     ncu = dev.info.num_cu
-    for nw in (1, 5, 64, ncu, 4 * ncu):
+    for nw in (1, 5, 8, 64, ncu, 4 * ncu, 8 * ncu):
         r = check(dev, ohelp, prim, 4, nw, 20, mode=mode, sync_scope=scope)
         assert r.sync_scope_used == scope
     check(dev, ohelp, prim, 2, 2 * ncu, 3, reps=2, mode=mode, wg=128, sync_scope=scope)
+
+
+@pytest.mark.parametrize("prim", BARRIER + SRB)
+@pytest.mark.parametrize("scope", [0, 1])
+def test_barrier_regions_return(dev, ohelp, prim, scope):
+    """More episodes per launch than WGs: every region passes through every
+    XCD's WGs and comes back (the XCD-aware walks keep a region in one L2
+    between hand-overs, so its return must not meet a stale copy)."""
+    ncu = dev.info.num_cu
+    for nw, iters in ((64, 200), (ncu, 3 * ncu + 5), (4 * ncu, 2 * 4 * ncu + 3)):
+        check(dev, ohelp, prim, 3, nw, iters, reps=2, mode=1, sync_scope=scope)
 
 
 @pytest.mark.parametrize("prim", BARRIER)
