@@ -196,15 +196,33 @@ __device__ __forceinline__ uint32_t meta_slot(uint32_t m) { return m >> 16; }
 __device__ __forceinline__ uint32_t meta_depth(uint32_t m) { return m & 0xffffu; }
 
 // wave-wide exclusive prefix sum of v (lanes 0..63)
+// Wave64 inclusive scans by DPP (GFX9 data-parallel primitives, no LDS
+// crossbar): a Kogge-Stone scan inside each 16-lane row (row_shr 1, 2, 4,
+// 8; lanes shifted in from outside the row read the identity), then row 0's
+// total (row_bcast:15) into rows 1 and 3, and rows 0-1's total
+// (row_bcast:31) into rows 2 and 3.  Every lane of the wave must be active.
+constexpr int kDppRowShr = 0x110, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+template <bool MAX>
+__device__ __forceinline__ uint32_t dpp_step(uint32_t x, uint32_t y) {
+  return MAX ? (x > y ? x : y) : x + y;
+}
+template <bool MAX>
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  const int id = 0;  // identity of + and of unsigned max
+  x = dpp_step<MAX>(x, __builtin_amdgcn_update_dpp(id, x, kDppRowShr + 1, 0xF, 0xF, false));
+  x = dpp_step<MAX>(x, __builtin_amdgcn_update_dpp(id, x, kDppRowShr + 2, 0xF, 0xF, false));
+  x = dpp_step<MAX>(x, __builtin_amdgcn_update_dpp(id, x, kDppRowShr + 4, 0xF, 0xF, false));
+  x = dpp_step<MAX>(x, __builtin_amdgcn_update_dpp(id, x, kDppRowShr + 8, 0xF, 0xF, false));
+  x = dpp_step<MAX>(x, __builtin_amdgcn_update_dpp(id, x, kDppRowBcast15, 0xA, 0xF, false));
+  x = dpp_step<MAX>(x, __builtin_amdgcn_update_dpp(id, x, kDppRowBcast31, 0xC, 0xF, false));
+  return x;
+}
+
+// Exclusive prefix sum over the wave; *total = the sum of all lanes.
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t *total) {
-  const int lane = threadIdx.x & 63;
-  uint32_t x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  *total = __shfl(x, 63, 64);
+  const uint32_t x = wave_incl_scan<false>(v);
+  *total = __builtin_amdgcn_readlane(x, 63);
   return x - v;
 }
 
@@ -212,7 +230,7 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t *total) 
 // one LDS update when the whole wave holds records of a single slot.
 __device__ __forceinline__ void slot_count_add(int *hold, int *cred, uint32_t sl, bool valid,
                                                int d) {
-  const uint32_t sl0 = __shfl(sl, 0, 64);
+  const uint32_t sl0 = __builtin_amdgcn_readlane(sl, 0);
   const unsigned long long vm = __ballot(valid);
   if (__all(!valid || sl == sl0)) {
     if ((threadIdx.x & 63) == 0 && vm) {
@@ -246,7 +264,8 @@ __device__ void dispatch(const UtsArgs &a, Guard &g) {
   uint32_t seen = 0;
   for (;;) {
     const uint32_t sub =
-        __shfl(lane == 0 ? __hip_atomic_load(a.submitted, __ATOMIC_ACQUIRE, sys) : 0u, 0, 64);
+        __builtin_amdgcn_readlane(
+            lane == 0 ? __hip_atomic_load(a.submitted, __ATOMIC_ACQUIRE, sys) : 0u, 0);
     if (sub != seen) {
       for (uint32_t i = seen + lane; i < sub; i += kWave)
         for (int w = 0; w < 3; ++w)
@@ -263,7 +282,7 @@ __device__ void dispatch(const UtsArgs &a, Guard &g) {
       if (!g.ok()) stop = 1;
       if (!stop) sleep_quanta(16);
     }
-    if (__shfl(stop, 0, 64)) return;
+    if (__builtin_amdgcn_readlane(stop, 0)) return;
   }
 }
 
@@ -455,7 +474,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
         }
       }
     }
-    v = __shfl(v, 0, 64);
+    v = __builtin_amdgcn_readlane(v, 0);
     if (v < 0) return;
     const int k = (int)std::min<long>((top - bot) / 2, (long)a.wbox_cap);
     uint32_t *box = a.wbox + (long)v * a.wbox_cap * W;
@@ -550,20 +569,17 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
       // per-tree accounting: one lane when the batch is all one tree (the
       // held count changes by total - n: children in, parents out)
       {
-        const uint32_t slot0 = __shfl(slot, 0, 64);
+        const uint32_t slot0 = __builtin_amdgcn_readlane(slot, 0);
         const bool active = lane < n;
         if (__all(!active || slot == slot0)) {
           const unsigned long long leafmask = __ballot(active && nc == 0);
-          uint32_t dmax = active ? dep : 0;
+          uint32_t dmax = 0;
           unsigned long long lsum = active ? lval : 0;
+          if (TREE == 1) {
+            dmax = __builtin_amdgcn_readlane(wave_incl_scan<true>(active ? dep : 0), 63);
+          } else {
 #pragma unroll
-          for (int o = 32; o > 0; o >>= 1) {
-            if (TREE == 1) {
-              const uint32_t od = __shfl_xor(dmax, o, 64);
-              dmax = od > dmax ? od : dmax;
-            } else {
-              lsum += __shfl_xor(lsum, o, 64);
-            }
+            for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
           }
           if (lane == 0) {
             s_nodes[slot0] += (unsigned long long)n;
@@ -642,7 +658,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
       phase(st_tgen);
       // ---- hand-offs: a late delivery in, the oldest half out ---------------
       {
-        const int k_in = (int)__shfl(box_n, 0, 64);
+        const int k_in = (int)__builtin_amdgcn_readlane(box_n, 0);
         if (k_in > 0 && !absorb(k_in)) return;
       }
       if ((batches & 1) == 0 && top - bot >= 2 * kWave) donate();
@@ -661,7 +677,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
     // ---- no local work: settle, then look for more -------------------------
     settle();
     {
-      const int k_in = (int)__shfl(lane == 0 ? ld_rlx(my_n) : 0u, 0, 64);
+      const int k_in = (int)__builtin_amdgcn_readlane(lane == 0 ? ld_rlx(my_n) : 0u, 0);
       if (k_in > 0) {
         if (!absorb(k_in)) return;
         continue;
@@ -749,18 +765,19 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   }
 }
 
-// One tree per launch.  The register budget admits three waves per SIMD
-// (168 VGPRs, no spills): a grid of that size hides the record loads of
-// one wave behind the SHA-1 work of the others on big trees.
+// One tree per launch.  The register budget admits four waves per SIMD
+// (at most 128 VGPRs; the kernel needs 116, no spills): a grid of that size
+// hides the record loads of one wave behind the SHA-1 work of the others on
+// big trees.
 template <int TREE>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 k_uts(UtsArgs a) {
   uts_worker<TREE, false, false>(a);
 }
 
 // The same with per-WG statistics (HS_UTS_STATS=1).
 template <int TREE>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 k_uts_stats(UtsArgs a) {
   uts_worker<TREE, false, true>(a);
 }
@@ -910,10 +927,11 @@ struct Engine {
   }
 
   // One geometric tree: 8 one-wave WGs per CU, or as many as can be resident
-  // (three per SIMD) when the tree is expected to be big.  Measured on
-  // MI355X (b0 = 4): the full grid wins from about 1e9 nodes (depth 14:
-  // 32.0 vs 28.7 G nodes/s; depth 15: 37.2 vs 31.3) and loses below (depth
-  // 12: 11.8 vs 14.7), where more WGs only add idle polling.
+  // (four per SIMD) when the tree is expected to be big.  Measured on
+  // MI355X (b0 = 4): the full grid wins from about 1e9 nodes (depth 15:
+  // 41.8 G nodes/s at 4096 WGs, 38.7 at 3072, 35.1 at 2048) and loses
+  // below (depth 12: 11.8 vs 14.7 G nodes/s at 3072 vs 2048), where more
+  // WGs only add idle polling.
   static int default_blocks(hs_ctx *c, double b0, int gen_mx) {
     if (std::pow(b0, (double)gen_mx) < 2.5e8) return 8 * c->num_cu;
     int fit = 0;
