@@ -48,20 +48,28 @@ def parse():
                     help="agent-scope release/acquire in every WG")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--cpu-episodes", type=int, default=1500)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU-baseline sample length (about this much wall time)")
     return ap.parse_args()
 
 
-def cpu_baseline(prim, L, nw, wg, ncu, episodes):
+def cpu_baseline(prim, L, nw, wg, ncu, seconds):
     """The oracle's threaded CPU port of the same barrier workload, on a
-    bounded sample (`episodes` episodes instead of 10000)."""
+    bounded sample: a 1500-episode run sets the rate, then about `seconds`
+    of episodes (at most 300000) are timed."""
     from tests.conftest import _load_oracle
     o = _load_oracle()
     threads = max(1, min(16, os.cpu_count() or 1))
     n = o.hso_num_storage_locs(prim, nw, ncu, wg, L)
     s = np.zeros(n, np.int32)
-    secs = o.hso_cpu_barrier_run(prim, L, nw, episodes, 1, wg, ncu, 0, threads,
-                                 s.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n)
+    ptr = s.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    probe = 1500
+    secs = o.hso_cpu_barrier_run(prim, L, nw, probe, 1, wg, ncu, 0, threads, ptr, n)
+    episodes = probe
+    if secs > 0 and seconds > 0:
+        episodes = int(min(300000, max(probe, seconds * probe / secs)))
+        s[:] = 0
+        secs = o.hso_cpu_barrier_run(prim, L, nw, episodes, 1, wg, ncu, 0, threads, ptr, n)
     want = np.zeros(n, np.int32)
     o.hso_golden_closed(prim, L, nw, episodes, 1, wg, ncu, 0,
                         want.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n)
@@ -70,7 +78,8 @@ def cpu_baseline(prim, L, nw, wg, ncu, episodes):
         "unit": "barrier episodes/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{episodes} episodes of the same {nw}-WG x {wg}-lane, numLdSt={L} "
+        "sample": (f"{episodes} episodes ({secs:.1f} s) of the same {nw}-WG x {wg}-lane, "
+                   f"numLdSt={L} "
                    f"workload on {threads} std::threads (each owns {nw // threads} "
                    f"emulated WGs, one std::atomic barrier per episode); "
                    f"result matches the closed form: {bool(np.array_equal(s, want))}"),
@@ -169,7 +178,7 @@ def main():
             r.cs_entries / (r.roi_ms / 1e3), 1)
         out["extras"] = extras
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(prim, a.num_ldst, nw, a.wg, ncu, a.cpu_episodes)
+        out["cpu_baseline"] = cpu_baseline(prim, a.num_ldst, nw, a.wg, ncu, a.cpu_seconds)
     dev.close()
     print(json.dumps(out))
 
