@@ -4,7 +4,7 @@
 //   2. agent-scope release / acquire cost vs. number of concurrent WGs
 //   3. one-way flag hand-off latency (same XCD / across XCDs)
 //   4. device-scope atomic RMW throughput: one address, one per XCD, one per
-//      WG, one per lane (the uncontended peak)
+//      WG, one per lane, eight per lane in turn (the uncontended peak)
 //   5. visibility: which store/load flavours observe each other across WGs
 //      on the same XCD and across XCDs
 //   6. a line's return to an XCD whose L2 an earlier store left it in
@@ -70,13 +70,20 @@ __global__ void k_pingpong(int a, int b, int rounds, uint32_t *flag, uint32_t *r
   }
 }
 
-// 0: one address; 1: one per XCD; 2: one per WG; 3: one per lane
+// 0: one address; 1: one per XCD; 2: one per WG; 3: one per lane; 4: eight
+// per lane in turn (consecutive adds of a lane hit different addresses, so
+// no address sees back-to-back RMWs from one lane)
 __global__ void k_atomic(int mode, int iters, uint32_t *ctr) {
   if (mode < 3 && threadIdx.x != 0) return;
   uint32_t *p = ctr;
   if (mode == 1) p = ctr + (blockIdx.x % 8) * 64;
   if (mode == 2) p = ctr + blockIdx.x * 64;
-  if (mode == 3) p = ctr + blockIdx.x * blockDim.x + threadIdx.x;
+  if (mode >= 3) p = ctr + blockIdx.x * blockDim.x + threadIdx.x;
+  if (mode == 4) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (int i = 0; i < iters; ++i) add_rlx(p + (i & 7) * stride, 1);
+    return;
+  }
   for (int i = 0; i < iters; ++i) add_rlx(p, 1);
 }
 
@@ -255,13 +262,14 @@ int main(int argc, char **argv) {
                 pr.second == 8 ? "same XCD" : "other XCD", ms * 1000 / (2 * rounds),
                 hr[2] ? " (TIMEOUT)" : "");
   }
-  for (int mode = 0; mode < 4; ++mode)
-    for (int nwg : {256, 1024}) {
-      const int it = mode == 3 ? 100 : 1000;
-      CHECK(hipMemset(buf, 0, 1 << 22));
+  for (int mode = 0; mode < 5; ++mode)
+    for (int nwg : {256, 1024}) {  // mode 4 spans 8 x 1024 x 256 words = 8 MiB of buf
+      const int it = mode >= 3 ? 100 : 1000;
+      CHECK(hipMemset(buf, 0, 1 << 24));
       const float ms = time_ms([&] { k_atomic<<<nwg, 256>>>(mode, it, buf); });
-      const double ops = (double)it * nwg * (mode == 3 ? 256 : 1);
-      static const char *kName[] = {"one address", "one per XCD", "one per WG", "one per lane"};
+      const double ops = (double)it * nwg * (mode >= 3 ? 256 : 1);
+      static const char *kName[] = {"one address", "one per XCD", "one per WG", "one per lane",
+                                    "8 per lane"};
       std::printf("atomic add %-12s %4d WGs: %9.3f G ops/s\n", kName[mode], nwg, ops / ms / 1e6);
     }
   for (int wmode = 0; wmode < 3; ++wmode)
