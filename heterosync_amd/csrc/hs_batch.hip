@@ -30,6 +30,7 @@ struct BInst {
   uint32_t *lk;
   uint32_t *ring;
   int ring_stride, prim, nw, iters, L, nab, per, vbase;
+  uint32_t sem_max;  // semaphores: the count a writer takes and returns
 };
 
 struct BArgs {
@@ -149,7 +150,7 @@ __global__ void __launch_bounds__(64) k_batch(BArgs a) {
           else mutex_unlock<kSleep>(lk, I.ring, I.ring_stride, s_ticket[j]);
           add_rlx(lk + kLkCs, 1);  // entry count, off the hand-off path
         } else {
-          sem_post(lk, writer, a.bo, g);
+          sem_post(lk, writer ? I.sem_max : 1u);
           add_rlx(lk + (writer ? kLkCs : kLkRd), 1);
         }
         s_bo[j] = a.bo.min_q;
@@ -225,6 +226,7 @@ int batch_prepare(hs_ctx *c, const hs_batch_instance *in, int n, int reps, Batch
       if ((rc = B.ar.alloc(&I.ring, (size_t)ring_len * I.ring_stride))) return rc;
     }
     const uint32_t sm = sem_max_of(p);
+    I.sem_max = sm;
     if (sm || I.ring) {
       const int blocks = (int)std::min<long>((ring_len + 255) / 256 + 1, 1024);
       k_batch_init<<<blocks, 256, 0, c->stream>>>(I.lk, sm, I.ring, ring_len, I.ring_stride);

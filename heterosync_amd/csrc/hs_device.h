@@ -42,6 +42,12 @@ __device__ __forceinline__ void st_rel(uint32_t *p, uint32_t v) {
 __device__ __forceinline__ uint32_t add_rlx(uint32_t *p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, HS_AGENT);
 }
+// Bookkeeping count bumped after a release: a non-returning add, so the
+// releaser does not wait on it, and increments from successive releasers
+// commute, so the total is exact once the kernel ends.
+__device__ __forceinline__ void count_rlx(uint32_t *p) {
+  (void)__hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, HS_AGENT);
+}
 __device__ __forceinline__ uint32_t add_rel(uint32_t *p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELEASE, HS_AGENT);
 }

@@ -85,7 +85,11 @@ __global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
   const bool writer = pw == cu % per;  // main.hip.cpp:883
   uint32_t *lk = a.locks + (UNIQ ? (long)cu * kLockWords : 0);
   sink[threadIdx.x] = 0;
-  if (is_master()) s_ok = !g.aborted();
+  uint32_t post_amount = 1u;  // a writer returns the whole count (max)
+  if (is_master()) {
+    s_ok = !g.aborted();
+    if (writer) post_amount = ld_rlx(lk + kLkMax);
+  }
   __syncthreads();
   if (!s_ok) return;
   for (int i = 0; i < a.iters; ++i) {
@@ -109,11 +113,9 @@ __global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
     if (AGENT) wait_mem();
     __syncthreads();
     if (is_master()) {
-      s_ok = sem_post<PRIO, AGENT>(lk, writer, a.bo, g);
+      sem_post<PRIO, AGENT>(lk, post_amount);
       add_rlx(lk + (writer ? kLkCs : kLkRd), 1);  // entry counts, off the hand-off path
     }
-    __syncthreads();
-    if (!s_ok) return;
   }
 }
 

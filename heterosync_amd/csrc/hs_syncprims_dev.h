@@ -400,21 +400,21 @@ __device__ __forceinline__ bool sem_wait(uint32_t *lk, bool writer,
   }
 }
 
-// Post: one release RMW (also publishes a writer's CS data).  The priority
-// form brackets it with the in-flight count that entering WGs yield to
-// (HSS:591-595, 639-640).
+// Post: one release RMW (also publishes a writer's CS data) that returns
+// `amount` to the count: the semaphore's max for a writer, 1 for a reader.
+// The caller reads max once (it never changes after initialisation), so
+// the release waits on no load.  The priority form brackets it with the
+// in-flight count that entering WGs yield to (HSS:591-595, 639-640).
 template <bool PRIO = false, bool AGENT = false>
-__device__ __forceinline__ bool sem_post(uint32_t *lk, bool writer,
-                                         const Backoff &, Guard &) {
+__device__ __forceinline__ void sem_post(uint32_t *lk, uint32_t amount) {
   if (PRIO) add_rlx(lk + kLkPrio, 1u);
   if (AGENT) {
     wait_mem();  // this wave's own stores (the CS counters) first
-    add_rlx(lk + kLkHead, writer ? ld_rlx(lk + kLkMax) : 1u);
+    add_rlx(lk + kLkHead, amount);
   } else {
-    add_rel(lk + kLkHead, writer ? ld_rlx(lk + kLkMax) : 1u);
+    add_rel(lk + kLkHead, amount);
   }
   if (PRIO) add_rlx(lk + kLkPrio, 0xffffffffu);
-  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -439,8 +439,8 @@ __device__ __forceinline__ bool atomic_global_arrive(const TreeBar &t,
   const uint32_t gg = ld_rlx(gen);
   if (add_acqrel(cnt, 1) == (uint32_t)t.nab - 1) {
     st_rlx(cnt, 0);
-    st_rlx(t.episodes, ld_rlx(t.episodes) + 1);
     st_rel(gen, gg + 1);
+    count_rlx(t.episodes);
     return true;
   }
   return spin_until_ne(gen, gg, poll_q, g);
@@ -517,8 +517,8 @@ __device__ __forceinline__ bool flag_tree_barrier(const FlagBar &f, int grp,
       for (int m = 1 + tid; m < f.nab; m += nthr)
         ok = ok && spin_until_eq(f.gin + m, e_global, poll_q, g);
       if (!__syncthreads_and(ok)) return false;
-      if (tid == 0) st_rlx(f.episodes, ld_rlx(f.episodes) + 1);
       for (int m = 1 + tid; m < f.nab; m += nthr) st_rel(f.gout + m, e_global);
+      if (tid == 0) count_rlx(f.episodes);
     }
   }
   if (f.per_cu > 1) {
@@ -643,9 +643,9 @@ __device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp, int q
   if ((add_rlx(sc, 1) + 1) % (uint32_t)ck != 0) return poll_eq_pipe(rel, epoch, poll_q, g);
   if ((add_rlx(t.glob, 1) + 1) % (uint32_t)t.n_sub != 0) return poll_eq_pipe(rel, epoch, poll_q, g);
   const uint32_t nx = (uint32_t)(t.nab < t.nxcd ? t.nab : t.nxcd);
-  st_rlx(t.episodes, ld_rlx(t.episodes) + 1);
-  st_rlx(t.glob + 32, epoch);
   for (uint32_t y = 0; y < nx; ++y) st_rlx(t.rel + (long)y * kBarWords, epoch);
+  st_rlx(t.glob + 32, epoch);
+  count_rlx(t.episodes);
   return true;
 }
 
@@ -739,8 +739,8 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
   }
   if (!__syncthreads_and(ok)) return false;
   const int nx = f.nab < f.nxcd ? f.nab : f.nxcd;
-  if (tid == 0) st_rlx(f.episodes, ld_rlx(f.episodes) + 1);
   for (int y = tid; y < nx; y += nthr) st_rlx(f.gout + (long)y * S, eg);
+  if (tid == 0) count_rlx(f.episodes);
   return true;
 }
 
@@ -767,8 +767,8 @@ __device__ __forceinline__ bool flat_arrive(uint32_t *blk, uint32_t weight, uint
       : __hip_atomic_fetch_add(cnt, (uint64_t)weight, __ATOMIC_ACQ_REL, HS_AGENT);
   const uint32_t e = (uint32_t)(old / parts);
   if ((old + weight) % parts == 0) {
-    if (episodes) st_rlx(episodes, ld_rlx(episodes) + 1);
     if (AGENT) st_rlx(gen, e + 1); else st_rel(gen, e + 1);
+    if (episodes) count_rlx(episodes);
     return true;
   }
   return AGENT ? poll_eq_pipe(gen, e + 1, poll_q, g) : spin_until_eq(gen, e + 1, poll_q, g);
