@@ -13,6 +13,7 @@
 //                               without local exchange
 //                               (main.hip.cpp:104-149, 315-373, 542-643)
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -329,6 +330,20 @@ __global__ void k_init_locks(uint32_t *locks, int nlocks, uint32_t sem_max,
   }
 }
 
+// WGs per sub-group of the XCD-aware barriers.  The counter tree costs
+// about (sub + arrivals at the device counter) same-address RMWs in series,
+// and the device counter takes grid / sub arrivals, so sub near the square
+// root of the grid balances the two levels (MI355X: 16 best at 256 and 512
+// WGs, 32 at 1024 and 2048).  The flag barrier's gathers poll one flag per
+// lane, which favours smaller sub-groups: the square root of half the grid
+// (16 at 1024 WGs, 32 at 2048).  A power of two in [16, 64].
+int sub_group_size(int grid, bool flags) {
+  const double r = std::sqrt(flags ? grid / 2.0 : (double)grid);
+  int s = 16;
+  while (s < 64 && 2 * s <= r) s *= 2;
+  return s;
+}
+
 bool is_barrier(int p) { return p < 4 || (p >= 37 && p <= 42); }
 bool is_mutex(int p) { return (p >= 4 && p <= 7) || (p >= 20 && p <= 23); }
 bool is_sem(int p) { return (p >= 8 && p <= 19) || (p >= 24 && p <= 35); }
@@ -480,11 +495,12 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   // barrier words, kBarWords blocks: [global][XCD-aware sub-counters:
   // nxcd x sub_max][groups: nab][release words: nxcd][episode count line]
   int sub_max = 0, n_sub = 0;  // XCD-aware tree: sub-counters per XCD, in all
+  const int sub = sub_group_size(nw, p == 2 || p == 3);
   if (xcd_mode)
     for (int x = 0; x < nxcd; ++x) {
       const int nwx = (nab / nxcd + (x < nab % nxcd ? 1 : 0)) * per;  // WGs on XCD x
-      sub_max = std::max(sub_max, (nwx + kSubArrive - 1) / kSubArrive);
-      n_sub += (nwx + kSubArrive - 1) / kSubArrive;
+      sub_max = std::max(sub_max, (nwx + sub - 1) / sub);
+      n_sub += (nwx + sub - 1) / sub;
     }
   const size_t o_sub = kBarWords, o_grp = o_sub + (size_t)kBarWords * nxcd * sub_max;
   const size_t o_rel = o_grp + (size_t)kBarWords * nab, o_ep = o_rel + (size_t)kBarWords * nxcd;
@@ -493,9 +509,9 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   // strict: packed flag arrays; XCD: one line per flag (kFlagStride)
   // flags: strict [gin: nab][gout: nab][lin, lout: nab x per], packed;
   // XCD-aware [gout: nxcd][din: nxcd x sub_max][sin: nxcd x sub_max x
-  // kSubArrive][lin, lout: nab x per], one line per flag
+  // sub][lin, lout: nab x per], one line per flag
   const size_t n_sub_slots = (size_t)nxcd * sub_max;
-  const size_t x_flags = nxcd + n_sub_slots * (1 + kSubArrive) + 2 * (size_t)nab * per;
+  const size_t x_flags = nxcd + n_sub_slots * (1 + sub) + 2 * (size_t)nab * per;
   const size_t flag_words = (xcd_mode ? x_flags * kFlagStride
                                       : 2 * (size_t)nab + 2 * (size_t)nab * per) +
                             4 * HS_LINE_WORDS;
@@ -534,15 +550,15 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   // XCD: global, nxcd XCD blocks, nab group blocks, nxcd release words
   a.tb = TreeBar{d_bar, d_bar + o_grp, episodes, nab, per};
   a.xt = XcdTree{d_bar, d_bar + o_sub, d_bar + o_grp, d_bar + o_rel, episodes, nab, per, nxcd,
-                 sub_max, n_sub};
+                 sub_max, n_sub, sub};
   uint32_t *fl = d_flags + 4 * HS_LINE_WORDS;
   a.fb = FlagBar{fl, fl + nab, fl + 2 * nab, fl + 2 * nab + (size_t)nab * per, episodes,
                  nab, per};
   {
     const size_t S = kFlagStride, o_din = nxcd, o_sin = o_din + n_sub_slots;
-    const size_t o_lin = o_sin + n_sub_slots * kSubArrive, o_lout = o_lin + (size_t)nab * per;
+    const size_t o_lin = o_sin + n_sub_slots * sub, o_lout = o_lin + (size_t)nab * per;
     a.xf = XcdFlags{fl, fl + o_din * S, fl + o_sin * S, fl + o_lin * S, fl + o_lout * S,
-                    episodes, nab, per, nxcd, sub_max};
+                    episodes, nab, per, nxcd, sub_max, sub};
   }
   a.flat = d_flat;
   a.nxcd = nxcd;
@@ -673,7 +689,7 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
         for (int x = 0; x < nx; ++x) {
           const int nwx = (nab / nxcd + (x < nab % nxcd ? 1 : 0)) * per;
           for (int k = 0; k < sub_max; ++k) {
-            const int ck = std::max(0, std::min(kSubArrive, nwx - k * kSubArrive));
+            const int ck = std::max(0, std::min(sub, nwx - k * sub));
             ok = ok && bar[o_sub + (size_t)kBarWords * (x * sub_max + k)] == te * (uint32_t)ck;
           }
           ok = ok && bar[o_rel + (size_t)kBarWords * x] == te;  // release word
@@ -704,14 +720,14 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
         for (int x = 0; x < nx; ++x) {
           ok = ok && gout[x * S] == te;
           const int nwx = (nab / nxcd + (x < nab % nxcd ? 1 : 0)) * per;
-          for (int k = 0; k * kSubArrive < nwx; ++k) {
+          for (int k = 0; k * sub < nwx; ++k) {
             const size_t sid = (size_t)x * sub_max + k;
             if (sid != 0) ok = ok && din[sid * S] == te;
-            const int ck = std::min(kSubArrive, nwx - k * kSubArrive);
-            for (int m = 1; m < ck; ++m) ok = ok && sinf[(sid * kSubArrive + m) * S] == te;
+            const int ck = std::min(sub, nwx - k * sub);
+            for (int m = 1; m < ck; ++m) ok = ok && sinf[(sid * sub + m) * S] == te;
           }
         }
-        lin = sinf + n_sub_slots * kSubArrive * S;
+        lin = sinf + n_sub_slots * sub * S;
         lin_want = lout_want = lx ? tl - 1 : 0;
         w1 = n_sub_slots > 1 ? din[S] : 0;
       } else {

@@ -593,12 +593,8 @@ __device__ __forceinline__ int groups_on_xcd(int nab, int nxcd, int x) {
   return nab / nxcd + (x < nab % nxcd ? 1 : 0);
 }
 
-// WGs per arrival counter of the XCD-aware tree: 16 balances the two
-// levels' serialisation at one address (about 11 ns per RMW) against one
-// more dependent round trip per level (1024 WGs: 64 counters of 16, then
-// one device counter of 64 arrivals).
-constexpr int kSubArrive = 16;
-
+// The XCD-aware barriers split each XCD's WGs into sub-groups of `sub`
+// (a power of two the host picks from the grid, sub_group_size()).
 struct XcdTree {
   uint32_t *glob;      // kBarWords: count @0, gen @32 (one arrival per sub-counter)
   uint32_t *sub;       // nxcd * sub_max * kBarWords: sub-counters, XCD-major
@@ -607,13 +603,14 @@ struct XcdTree {
   uint32_t *episodes;
   int nab, per_cu, nxcd;
   int sub_max, n_sub;  // sub-counters per XCD (at most) and in all
+  int sub_wgs;         // WGs per sub-counter
 };
 
 // Master thread; every wave of the WG has run wait_mem() + __syncthreads().
 // A global arrival climbs two levels of relaxed RMWs on counters that are
 // never reset (the last arriver of a level is the one whose increment
 // completes a multiple of the level's size): the WG's sub-counter (up to
-// kSubArrive WGs of one XCD), then the device counter (one arrival per
+// sub_wgs WGs of one XCD), then the device counter (one arrival per
 // sub-counter).  The device releaser writes one release word per XCD and
 // every WG of that XCD polls its word, so the release is a single hop.
 // `q` is the WG's index in its group, so (grp / nxcd, q) numbers the WGs of
@@ -637,8 +634,8 @@ __device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp, int q
   uint32_t *rel = t.rel + (long)x * kBarWords;
   const int nwx = groups_on_xcd(t.nab, t.nxcd, x) * (int)per;  // WGs on this XCD
   const int wx = (grp / t.nxcd) * (int)per + q;
-  const int k = wx / kSubArrive;
-  const int ck = nwx - k * kSubArrive < kSubArrive ? nwx - k * kSubArrive : kSubArrive;
+  const int sw = t.sub_wgs, k = wx / sw;
+  const int ck = nwx - k * sw < sw ? nwx - k * sw : sw;
   uint32_t *sc = t.sub + (long)(x * t.sub_max + k) * kBarWords;
   if ((add_rlx(sc, 1) + 1) % (uint32_t)ck != 0) return poll_eq_pipe(rel, epoch, poll_q, g);
   if ((add_rlx(t.glob, 1) + 1) % (uint32_t)t.n_sub != 0) return poll_eq_pipe(rel, epoch, poll_q, g);
@@ -657,17 +654,18 @@ constexpr int kFlagStride = HS_LINE_WORDS;
 struct XcdFlags {
   uint32_t *gout;        // nxcd: release word per XCD
   uint32_t *din;         // nxcd * sub_max: sub-group leaders' arrivals (device level)
-  uint32_t *sin;         // nxcd * sub_max * kSubArrive: members' arrivals (sub level)
+  uint32_t *sin;         // nxcd * sub_max * sub_wgs: members' arrivals (sub level)
   uint32_t *lin, *lout;  // nab * per_cu: group level (local exchange), group-major
   uint32_t *episodes;
   int nab, per_cu, nxcd;
   int sub_max;           // sub-groups per XCD (at most), as for XcdTree
+  int sub_wgs;           // WGs per sub-group
 };
 
 // Whole-WG call.  el = group-level epoch, eg = episode (device level).
 // The global barrier gathers arrival flags on two levels, as the tree
 // barrier counts them: the WGs of one XCD form sub-groups of up to
-// kSubArrive (numbered as in xcd_tree_arrive), each leader gathers its
+// sub_wgs (numbered as in xcd_tree_arrive), each leader gathers its
 // members' flags one per lane, and the device master (leader of sub-group 0)
 // gathers the leaders' flags one per lane.  The release is single-hop: the
 // device master writes one release word per XCD (gout[x]) and every WG of
@@ -704,10 +702,10 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
   const int per = f.per_cu;
   const int nwx = groups_on_xcd(f.nab, f.nxcd, x) * per;  // WGs on this XCD
   const int wx = (grp / f.nxcd) * per + q;
-  const int k = wx / kSubArrive, m = wx % kSubArrive;
-  const int ck = nwx - k * kSubArrive < kSubArrive ? nwx - k * kSubArrive : kSubArrive;
+  const int sw = f.sub_wgs, k = wx / sw, m = wx % sw;
+  const int ck = nwx - k * sw < sw ? nwx - k * sw : sw;
   const long sid = (long)x * f.sub_max + k;
-  uint32_t *sin = f.sin + sid * kSubArrive * S;
+  uint32_t *sin = f.sin + sid * sw * S;
   if (m != 0) {  // member: arrive at the sub level, wait for the release
     if (tid == 0) {
       st_rlx(sin + m * S, eg);
@@ -734,7 +732,7 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
   ok = true;
   for (int y = 1 + tid; y < nsub; y += nthr) {
     const int xy = y / f.sub_max, ky = y % f.sub_max;
-    if (ky * kSubArrive < groups_on_xcd(f.nab, f.nxcd, xy) * per)  // sub-group exists
+    if (ky * sw < groups_on_xcd(f.nab, f.nxcd, xy) * per)  // sub-group exists
       ok = ok && poll_eq(f.din + (long)y * S, eg, poll_q, g);
   }
   if (!__syncthreads_and(ok)) return false;

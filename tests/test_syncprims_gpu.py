@@ -91,7 +91,8 @@ def test_uniq_semaphore(dev, ohelp, prim, scope):
 @pytest.mark.parametrize("scope", [0, 1])
 def test_barrier_small(dev, ohelp, prim, mode, scope):
     ncu = dev.info.num_cu
-    for nw in (1, 5, 8, 64, ncu, 4 * ncu, 8 * ncu):
+    # up to 16 WGs per CU: the XCD-aware sub-groups take 16, 32 and 64 WGs
+    for nw in (1, 5, 8, 64, ncu, 4 * ncu, 8 * ncu, 16 * ncu):
         r = check(dev, ohelp, prim, 4, nw, 20, mode=mode, sync_scope=scope)
         assert r.sync_scope_used == scope
     check(dev, ohelp, prim, 2, 2 * ncu, 3, reps=2, mode=mode, wg=128, sync_scope=scope)
@@ -123,7 +124,7 @@ def test_barrier_full_config(dev, ohelp, prim, scope):
 def test_srb_small(dev, ohelp, prim, mode, scope):
     """Sense-reversing family: flat below 5 WGs per group, tree above."""
     ncu = dev.info.num_cu
-    for nw in (1, 5, 64, ncu, 2 * ncu, 4 * ncu, 8 * ncu):
+    for nw in (1, 5, 64, ncu, 2 * ncu, 4 * ncu, 8 * ncu, 16 * ncu):
         r = check(dev, ohelp, prim, 3, nw, 12, reps=3, mode=mode, sync_scope=scope)
         assert r.sync_scope_used == srb_scope(prim, nw, ncu, scope)
     check(dev, ohelp, prim, 2, 2 * ncu, 3, reps=2, mode=mode, wg=128, sync_scope=scope)
