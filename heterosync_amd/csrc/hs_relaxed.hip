@@ -311,11 +311,14 @@ __global__ void __launch_bounds__(1024) k_refcounter(RefArgs a) {
 // ---------------------------------------------------------------------------
 // splitCounter (CSK:10-103), wave-granular: even waves of a WG are writers
 // (one relaxed atomicAdd(+1) per iteration on their own counter), odd waves
-// are readers.  Every `read_every` iterations a reader wave loads its WG's
-// writer counters, reduces them across the wave and publishes the WG's
-// partial sum; it then reads all partials (a strided sweep shared by the
-// lanes) for the approximate global total.  Counters are exact at the end
-// (iters each); reader totals lie in [0, total].
+// are readers.  The reference's read is WG-collective: the reader threads
+// of a TB share one sweep and all output its sum (CSK:33-76, 95).  Here the
+// first reader wave makes the WG's read every `read_every` iterations: it
+// loads the WG's writer counters, reduces them across the wave and
+// publishes the WG's partial sum, then reads all partials (a strided sweep
+// shared by the lanes) for the approximate global total, which every
+// reader thread outputs at the end.  Counters are exact at the end (iters
+// each); reader totals lie in [0, total].
 struct SplitArgs {
   uint32_t *counters;  // num_wgs * wg/2
   uint32_t *partial;   // num_wgs
@@ -329,30 +332,45 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
+// Sum of p[lane], p[lane + 64], ... below n, by relaxed loads issued eight at
+// a time before any is added: the words are independent and the sum is an
+// approximate snapshot anyway, so the loads need not wait for each other (a
+// load-add loop waits out one memory round trip per word).
+__device__ __forceinline__ uint32_t sweep_sum(const uint32_t *p, int n, int lane) {
+  uint32_t s = 0;
+  for (int k = lane; k < n; k += 8 * 64) {
+    uint32_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = k + u * 64 < n ? ld_rlx(p + k + u * 64) : 0u;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  return s;
+}
+
 __global__ void __launch_bounds__(1024) k_splitcounter(SplitArgs a) {
   const int b = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int half = blockDim.x >> 1;  // writer threads per WG
   uint32_t *mine = a.counters + (long)b * half;
   const int nw = gridDim.x;
+  __shared__ uint32_t s_total;
   uint64_t ops = 0;
   if ((wave & 1) == 0) {
     uint32_t *c = mine + (wave >> 1) * 64 + lane;
     for (int i = 0; i < a.iters; ++i) add_rlx(c, 1u);
     ops = a.iters;
-  } else {
+  } else if (wave == 1) {  // the WG's read, made once (CSK:33-76 is WG-collective)
     uint32_t total = 0;
     for (int i = a.read_every; i <= a.iters; i += a.read_every) {
-      uint32_t s = 0;
-      for (int k = lane; k < half; k += 64) s += ld_rlx(mine + k);
-      s = wave_sum(s);
-      if (lane == 0 && wave == 1) st_rlx(a.partial + b, s);
-      uint32_t tot = 0;
-      for (int k = lane; k < nw; k += 64) tot += ld_rlx(a.partial + k);
-      total = wave_sum(tot);
+      const uint32_t s = wave_sum(sweep_sum(mine, half, lane));
+      if (lane == 0) st_rlx(a.partial + b, s);
+      total = wave_sum(sweep_sum(a.partial, nw, lane));
       ops += (half + 63) / 64 + (nw + 63) / 64 + 1;
     }
-    a.out[(long)b * blockDim.x + threadIdx.x] = total;
+    if (lane == 0) s_total = total;
   }
+  __syncthreads();
+  if (wave & 1) a.out[(long)b * blockDim.x + threadIdx.x] = s_total;  // every reader thread
   if (lane == 0)
     __hip_atomic_fetch_add(a.stats + 1, ops * 64, __ATOMIC_RELAXED, HS_AGENT);
 }
