@@ -216,8 +216,16 @@ __device__ __forceinline__ void wg_read(const int32_t *s, long tid0, int L,
   const long tid = tid0 + threadIdx.x;
   const int32_t *col = s + (tid / kWordsPerLine) * (L + 1) * kWordsPerLine +
                        threadIdx.x % kWordsPerLine;
+  // independent loads, issued eight at a time before any is added (a
+  // load-add loop over agent-scope loads waits out one round trip per line)
   int32_t acc = 0;
-  for (int n = L - 1; n >= 0; --n) acc += ld_data<SC1>(col + n * kWordsPerLine);
+  for (int n0 = 0; n0 < L; n0 += 8) {
+    int32_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = n0 + u < L ? ld_data<SC1>(col + (n0 + u) * kWordsPerLine) : 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
   sink[threadIdx.x] += acc;
 }
 
