@@ -216,12 +216,20 @@ __global__ void __launch_bounds__(1024) k_flags(FlagsArgs a) {
       ++s_ops;
     }
     const int dummy = a.accs / 4;
+    // Neither load of an iteration waits in line: the random value is read
+    // one iteration ahead (it is input), and the stop flag's load is issued
+    // before the iteration's work and consumed after it.
+    float r_next = a.iters > 0 ? a.rnd[b] : 1.0f;
     for (int it = 0;; ++it) {
       __syncthreads();
       const bool go = (!s_stop || it < a.iters) && s_ok;
       __syncthreads();  // all waves have read s_stop/s_ok
       if (!go) break;
-      if (it < a.iters && a.rnd[(long)b + (long)it * grid] <= a.cutoff) {
+      const float r = r_next;
+      if (it + 1 < a.iters) r_next = a.rnd[(long)b + (long)(it + 1) * grid];
+      uint32_t stop_v = 0;
+      if (is_master()) stop_v = ld_rlx(a.stop);
+      if (it < a.iters && r <= a.cutoff) {
         for (int i = 0; i < dummy; ++i) lds[threadIdx.x]++;
         __syncthreads();
         if (is_master()) {
@@ -233,7 +241,10 @@ __global__ void __launch_bounds__(1024) k_flags(FlagsArgs a) {
         sleep_quanta(bo >> 3);
       }
       if (is_master()) {
-        s_stop = ld_rlx(a.stop) != 0;
+        // an opaque use here, so the compiler does not fold the test (and
+        // the wait for the load) into the branch that issued it
+        asm volatile("" : "+v"(stop_v));
+        s_stop = stop_v != 0;
         ++s_ops;
         if (!s_stop && it >= a.iters && !g.ok()) s_ok = 0;
       }
