@@ -11,6 +11,12 @@
 // slot check or semaphore CAS) and moves on, so a spinning reader can never
 // starve the writer it waits for, whatever the residency.  Backoff becomes
 // "skip this virtual WG for N rounds".
+// Ordering is the plain form's fence-free one: critical-section data goes
+// through agent-scope accesses (L1 bypassed, stores written through), every
+// lane drains them before the barrier that precedes a release, and the lock
+// and semaphore words are relaxed atomics, so no acquire or release needs a
+// cache fence (the fenced protocol's release wrote back up to 25 KB of dirty
+// L2 per critical section).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -99,7 +105,7 @@ __global__ void __launch_bounds__(64) k_batch(BArgs a) {
         } else if (!sem) {
           uint32_t *lk = I.lk;
           if (p == 4 || p == 5) {  // spin / EBO: one test-and-test-and-set
-            enter = ld_rlx(lk + kLkHead) == 0 && cas_acq(lk + kLkHead, 0, 1) == 0;
+            enter = ld_rlx(lk + kLkHead) == 0 && cas_rlx(lk + kLkHead, 0, 1) == 0;
             if (!enter && p == 5) {
               s_skip[j] = s_bo[j] >> 2;
               s_bo[j] = ((s_bo[j] << 1) + 1) & a.bo.max_q;
@@ -110,17 +116,15 @@ __global__ void __launch_bounds__(64) k_batch(BArgs a) {
               s_phase[j] = kWaiting;
             }
             enter = ld_rlx(lk + kLkTail) == s_ticket[j];
-            if (enter) fence_acq();
           } else {  // p == 6, queue lock: take a slot, then check it
             if (s_phase[j] == kIdle) {
               s_ticket[j] = add_rlx(lk + kLkTail, 1);
               s_phase[j] = kWaiting;
             }
             enter = ld_rlx(I.ring + (long)s_ticket[j] * I.ring_stride) == 1u;
-            if (enter) fence_acq();
           }
         } else {
-          const int r = sem_try_wait(I.lk, writer, g);
+          const int r = sem_try_wait<true>(I.lk, writer, g);
           if (r < 0) s_ok = 0;
           enter = r > 0;
           if (!enter && p >= 12) {  // EBO semaphore
@@ -135,22 +139,23 @@ __global__ void __launch_bounds__(64) k_batch(BArgs a) {
       if (!s_enter) continue;
       // ---- critical section of virtual WG b of instance I --------------
       if (!sem) {
-        wg_access<0>(I.storage, 0, I.L);
+        wg_access<0, true>(I.storage, 0, I.L);
       } else if (writer) {
-        for (int k = 0; k < I.per; ++k) wg_access<0>(I.storage, (long)k * 64, I.L);
+        for (int k = 0; k < I.per; ++k) wg_access<0, true>(I.storage, (long)k * 64, I.L);
       } else {
-        wg_read(I.storage, (long)pw * 64, I.L, sink);
+        wg_read<true>(I.storage, (long)pw * 64, I.L, sink);
       }
+      wait_mem();  // every lane's CS accesses complete before the release
       __syncthreads();
       if (lane == 0) {
         uint32_t *lk = I.lk;
         if (!sem) {
-          if (p == 4 || p == 5) mutex_unlock<kSpin>(lk, nullptr, 0, 0);
-          else if (p == 7) mutex_unlock<kFA>(lk, nullptr, 0, s_ticket[j]);
-          else mutex_unlock<kSleep>(lk, I.ring, I.ring_stride, s_ticket[j]);
+          if (p == 4 || p == 5) mutex_unlock<kSpin, true>(lk, nullptr, 0, 0);
+          else if (p == 7) mutex_unlock<kFA, true>(lk, nullptr, 0, s_ticket[j]);
+          else mutex_unlock<kSleep, true>(lk, I.ring, I.ring_stride, s_ticket[j]);
           add_rlx(lk + kLkCs, 1);  // entry count, off the hand-off path
         } else {
-          sem_post(lk, writer ? I.sem_max : 1u);
+          sem_post<false, true>(lk, writer ? I.sem_max : 1u);
           add_rlx(lk + (writer ? kLkCs : kLkRd), 1);
         }
         s_bo[j] = a.bo.min_q;
