@@ -1,14 +1,18 @@
 """CPU tests that pin the oracle (oracle/hs_oracle.c, oracle/cpu_emul.cpp).
 
 Each function of the oracle is checked against something it does not
-restate: published known-answer vectors, an independent implementation
-(hashlib, a Python glibc-rand restatement), brute force over the kernels'
-access pattern, or a real concurrent run (std::thread emulation).  Planted
-mistakes (a wrong pass count, a dropped line) must be caught.
+restate: published known-answer vectors and the system glibc's own rand()
+output (fixtures in tests/golden, written by make_fixtures.py), an
+independent implementation (hashlib, a Python glibc-rand restatement),
+brute force over the kernels' access pattern, or a real concurrent run
+(std::thread emulation).  Planted mistakes (a wrong pass count, a dropped
+line) must be caught.
 """
 import ctypes
 import hashlib
+import json
 import math
+import os
 import random
 
 import numpy as np
@@ -16,6 +20,16 @@ import pytest
 
 P8 = ctypes.POINTER(ctypes.c_uint8)
 P32 = ctypes.POINTER(ctypes.c_int32)
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def fixture(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+def _fips_message(spec):
+    return spec.encode() if isinstance(spec, str) else (spec["repeat"] * spec["times"]).encode()
 
 SUPPORTED = list(range(36)) + list(range(37, 43))
 # priority EBO semaphores: the reference dispatches them to the global kernel
@@ -48,15 +62,10 @@ def sha1(oracle, msg: bytes) -> bytes:
 
 # ---- SHA-1 (FIPS 180-4 known answers) --------------------------------------
 
-@pytest.mark.parametrize("msg,digest", [
-    (b"abc", "a9993e364706816aba3e25717850c26c9cd0d89d"),
-    (b"", "da39a3ee5e6b4b0d3255bfef95601890afd80709"),
-    (b"abcdbcdecdefdefgefghfghighijhijkijkljklmklmnlmnomnopnopq",
-     "84983e441c3bd26ebaae4aa1f95129e5e54670f1"),
-    (b"a" * 1000000, "34aa973cd4c4daa4f61eeb2bdbad27316534016f"),
-])
-def test_sha1_known_answers(oracle, msg, digest):
-    assert sha1(oracle, msg).hex() == digest
+@pytest.mark.parametrize("vec", fixture("sha1_fips180.json")["vectors"],
+                         ids=lambda v: str(v["message"])[:12])
+def test_sha1_known_answers(oracle, vec):
+    assert sha1(oracle, _fips_message(vec["message"])).hex() == vec["sha1"]
 
 
 def test_sha1_matches_hashlib_all_lengths(oracle):
@@ -83,9 +92,12 @@ def test_uts_rng_definition(oracle):
 def test_uts_T1_published_sample_tree(oracle):
     """UTS sample tree T1 (-t 1 -a 3 -d 10 -b 4 -r 19): 4130071 nodes,
     3305118 leaves, depth 10 -- the public UTS distribution's figures."""
+    t1 = fixture("uts_t1.json")
+    pr = t1["params"]
     n, lv, md = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
-    oracle.hso_uts_geo_count(19, 4.0, 10, ctypes.byref(n), ctypes.byref(lv), ctypes.byref(md))
-    assert (n.value, lv.value, md.value) == (4130071, 3305118, 10)
+    oracle.hso_uts_geo_count(pr["root_seed"], pr["b0"], pr["gen_mx"], ctypes.byref(n),
+                             ctypes.byref(lv), ctypes.byref(md))
+    assert (n.value, lv.value, md.value) == (t1["nodes"], t1["leaves"], t1["max_depth"])
 
 
 def _py_children(h, b0, depth, gen_mx):
@@ -162,6 +174,19 @@ def _glibc_rand(seed, n):
         if k - 1 >= 344:
             out.append(v >> 1)
     return out[:n]
+
+
+@pytest.mark.parametrize("seed", [2018, 200])
+def test_glibc_rand_matches_the_system_glibc(oracle, seed):
+    """The oracle's generator and the Python restatement against the system
+    glibc's own rand() (fixture): the flags input (srand(2018), flags.cu:92-101)
+    and the UTS reference tree (srand(200), uts.cu:290)."""
+    real = fixture("glibc_rand.json")[f"seed_{seed}"]
+    assert _glibc_rand(seed, len(real)) == real
+    arr = np.zeros(len(real), np.float32)
+    oracle.hso_flags_rand(arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), arr.size, seed)
+    want = np.array(real, np.float32) / np.float32(4294967295.0)  # flags.cu:100
+    assert np.array_equal(arr, want)
 
 
 def test_flags_rand_is_glibc(oracle):
