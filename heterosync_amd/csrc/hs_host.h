@@ -89,4 +89,9 @@ inline uint64_t timeout_ticks(const hs_ctx *c, double s) {
 // Max co-resident WGs of `kernel` at `wg` threads per WG on this device.
 int coresident_wgs(const hs_ctx *c, const void *kernel, int wg, int *out);
 
+// Makes HIP load `kernel`'s code object now.  HIP loads a module on the
+// first use of one of its kernels; a timed region that begins with that
+// first launch would include the load (about 3 ms measured on MI355X).
+int load_kernel(const void *kernel);
+
 }  // namespace hs

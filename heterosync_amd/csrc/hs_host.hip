@@ -67,6 +67,12 @@ int coresident_wgs(const hs_ctx *c, const void *kernel, int wg, int *out) {
   return HS_OK;
 }
 
+int load_kernel(const void *kernel) {
+  int per_cu = 0;  // the occupancy query needs the loaded code object
+  HS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0));
+  return HS_OK;
+}
+
 }  // namespace hs
 
 extern "C" {

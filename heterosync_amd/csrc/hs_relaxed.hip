@@ -551,6 +551,9 @@ extern "C" int hs_refcounter_run(hs_ctx *c, int sharers, int num_tbs, int tb, in
   HS_HIP(hipMemset(d_del, 0, 2 * (size_t)nthr * 4));
   RefArgs a{d_c, d_c + ncount, d_del, d_del + nthr, d_stats, num_tbs / sharers, repeats,
             spacing < 0 ? 2 * repeats : spacing};
+  if ((rc = load_kernel(strict ? (const void *)k_refcounter<true>
+                               : (const void *)k_refcounter<false>)))
+    return rc;  // not inside the timed launch
   Timer t;
   HS_HIP(hipDeviceSynchronize());
   HS_HIP(hipEventRecord(t.e0, c->stream));
@@ -594,6 +597,7 @@ extern "C" int hs_splitcounter_run(hs_ctx *c, int num_tbs, int tb, int iters,
   HS_HIP(hipMemset(d_stats, 0, 32));
   const int read_every = iters >= 64 ? 64 : (iters > 0 ? iters : 1);
   SplitArgs a{d_cnt, d_part, d_out, d_stats, iters, read_every};
+  if ((rc = load_kernel((const void *)k_splitcounter))) return rc;  // not inside the timed launch
   Timer t;
   HS_HIP(hipDeviceSynchronize());
   HS_HIP(hipEventRecord(t.e0, c->stream));

@@ -577,6 +577,7 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   const uint32_t local_per_iter = (lx && per > 1) ? 2 : 1;
 
   // ---- region of interest (main.hip.cpp:2354-2511) -----------------------
+  if ((rc = load_kernel((const void *)kfn))) return rc;  // not inside the ROI
   EventSet ev;  // [0] ROI start, [1] ROI end, then a pair per launch
   if ((rc = ev.create(2 + 2 * (size_t)reps))) return rc;
   const hipEvent_t ev0 = ev[0], ev1 = ev[1];
