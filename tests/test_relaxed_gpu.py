@@ -43,6 +43,8 @@ def test_seqlock(dev, oracle, variant, nw, wg, gs, period, iters):
     (160, 64, 16, 4, 0.33, 1),      # the reference's run file (tbSize 32 -> wave64)
     (256, 64, 16, 4, 0.2, 100),
     (1024, 256, 16, 4, 0.33, 200),
+    (64, 64, 16, 4, 0.45, 2),       # the next value is read one iteration ahead
+    (1024, 256, 16, 4, 0.33, 10000),  # C4
 ])
 def test_flags(dev, oracle, ntb, tb, accs, reps, cut, iters):
     rnd = np.zeros(ntb * iters, np.float32)
@@ -78,7 +80,8 @@ def test_refcounter(dev, sh, ntb, tb, reps, spacing, strict):
     assert r.atomic_ops == 4 * ntb * tb * reps
 
 
-@pytest.mark.parametrize("ntb,tb,iters", [(112, 128, 1), (256, 256, 1000), (1024, 256, 200)])
+@pytest.mark.parametrize("ntb,tb,iters", [(112, 128, 1), (256, 256, 1000), (1024, 256, 200),
+                                         (64, 1024, 300)])
 def test_splitcounter(dev, ntb, tb, iters):
     counters, out, r = dev.splitcounter(ntb, tb, iters)
     assert np.all(counters == iters)
@@ -86,3 +89,7 @@ def test_splitcounter(dev, ntb, tb, iters):
     assert out.max() <= total
     w = out.reshape(ntb, tb // 64, 64)
     assert np.all(w[:, 0::2, :] == 0)  # writer waves write no output
+    # the read is WG-collective (CSK:33-76): every reader thread of a WG
+    # outputs the same sum
+    readers = w[:, 1::2, :].reshape(ntb, -1)
+    assert np.all(readers == readers[:, :1])
