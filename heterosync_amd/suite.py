@@ -52,7 +52,10 @@ def c2(dev, oh, scopes=(0, 1)):
             s, r = dev.syncprim(pid, 10, 4 * ncu, 1000, wg_size=256, reps=10, timeout_s=30.0,
                                 sync_scope=sc)
             ok = bool(np.array_equal(s, oh.golden(pid, 10, 4 * ncu, 1000, 10, 256, ncu, 0)))
-            out[f"{name}_{'strict' if sc else 'xcd'}"] = {
+            # named for the form that ran (a failed XCD placement check falls
+            # back to the strict form)
+            form = "strict" if r.sync_scope_used else "xcd"
+            out[f"{name}_{form}{'' if r.sync_scope_used == sc else '_fallback'}"] = {
                 "episodes_per_s": r.episodes / (r.roi_ms / 1e3),
                 "parity_ok": ok and r.state_ok == 1}
     return out
