@@ -295,6 +295,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   __shared__ uint32_t s_pst[W * kWave];   // parent records of the batch
   __shared__ uint32_t s_top[W * kWave];   // newest children, word-major, not in the deque
   __shared__ uint32_t s_off[kWave + 1];   // exclusive scan of child counts
+  __shared__ uint32_t s_first[kWave];     // per round: parent + 1 at its first child
   __shared__ int s_hold[kMaxSlots];       // nodes of each slot held by this WG
   __shared__ int s_cred[kMaxSlots];       // credits of each slot held by this WG
   __shared__ unsigned long long s_nodes[kMaxSlots], s_leaves[kMaxSlots], s_lsum[kMaxSlots];
@@ -611,15 +612,25 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
       // the newest min(total, 64) children stay in s_top (slot c - keep0)
       const uint32_t keep = total < (uint32_t)kWave ? total : (uint32_t)kWave;
       const uint32_t keep0 = total - keep;
+      uint32_t carry = 0;  // parent of the last child of the previous round
       for (uint32_t base = 0; base < total; base += kWave) {
         ++st_rounds;
+        // Parent of child c: the last parent with children whose first child
+        // is <= c.  Each such parent marks its first child's position in this
+        // round (parents with children start at distinct offsets), and an
+        // inclusive max-scan carries the marks forward; positions before the
+        // round's first mark belong to the previous round's last parent.
+        s_first[lane] = 0;
+        __syncthreads();
+        if (lane < n && nc > 0 && off >= base && off < base + (uint32_t)kWave)
+          s_first[off - base] = (uint32_t)lane + 1;
+        __syncthreads();
+        const uint32_t mark = wave_incl_scan<true>(s_first[lane]);
+        const int lo = mark ? (int)mark - 1 : (int)carry;
+        carry = (uint32_t)__builtin_amdgcn_readlane(lo, 63);
+        __syncthreads();  // s_first is read before the next round clears it
         const uint32_t c = base + lane;
         if (c < total) {
-          int lo = 0, hi = n - 1;  // parent p: s_off[p] <= c < s_off[p+1]
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= c) lo = mid; else hi = mid - 1;
-          }
           const uint32_t *pr = &s_pst[lo * W];
           const long at = (top + c) & dq_mask;
           if (TREE == 1) {
