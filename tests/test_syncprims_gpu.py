@@ -109,6 +109,16 @@ def test_barrier_regions_return(dev, ohelp, prim, scope):
         check(dev, ohelp, prim, 3, nw, iters, reps=2, mode=1, sync_scope=scope)
 
 
+@pytest.mark.parametrize("prim", BARRIER + SRB)
+@pytest.mark.parametrize("scope", [0, 1])
+def test_barrier_large_wgs(dev, ohelp, prim, scope):
+    """512- and 1024-thread WGs: a WG's region spans 32 or 64 chains, so the
+    band walk's lanes cover many chains each."""
+    ncu = dev.info.num_cu
+    check(dev, ohelp, prim, 10, ncu, 30, reps=2, wg=1024, mode=1, sync_scope=scope)
+    check(dev, ohelp, prim, 7, 2 * ncu, 30, reps=2, wg=512, mode=0, sync_scope=scope)
+
+
 @pytest.mark.parametrize("prim", BARRIER)
 @pytest.mark.parametrize("scope", [0, 1])
 def test_barrier_full_config(dev, ohelp, prim, scope):
