@@ -109,6 +109,16 @@ def test_barrier_regions_return(dev, ohelp, prim, scope):
         check(dev, ohelp, prim, 3, nw, iters, reps=2, mode=1, sync_scope=scope)
 
 
+@pytest.mark.parametrize("prim", MUTEX + UMUTEX + SEM + USEM)
+@pytest.mark.parametrize("scope", [0, 1])
+def test_locks_large_wgs(dev, ohelp, prim, scope):
+    """256- and 1024-thread WGs: the critical section's band walk spans 16
+    or 64 chains per WG."""
+    ncu = dev.info.num_cu
+    check(dev, ohelp, prim, 10, 16, 4, reps=2, wg=1024, mode=1, sync_scope=scope)
+    check(dev, ohelp, prim, 5, 2 * ncu, 3, reps=2, wg=256, mode=0, sync_scope=scope)
+
+
 @pytest.mark.parametrize("prim", BARRIER + SRB)
 @pytest.mark.parametrize("scope", [0, 1])
 def test_barrier_large_wgs(dev, ohelp, prim, scope):
