@@ -729,7 +729,8 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
           }
         }
         lin = sinf + n_sub_slots * sub * S;
-        lin_want = lout_want = lx ? tl - 1 : 0;
+        lin_want = lx ? tl - 1 : 0;  // the group barrier is all-to-all: no out-flags
+        lout_want = 0;
         w1 = n_sub_slots > 1 ? din[S] : 0;
       } else {
         const uint32_t *gin = f, *gout = f + nab;

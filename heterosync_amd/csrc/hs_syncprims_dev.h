@@ -663,7 +663,8 @@ struct XcdFlags {
   uint32_t *gout;        // nxcd: release word per XCD
   uint32_t *din;         // nxcd * sub_max: sub-group leaders' arrivals (device level)
   uint32_t *sin;         // nxcd * sub_max * sub_wgs: members' arrivals (sub level)
-  uint32_t *lin, *lout;  // nab * per_cu: group level (local exchange), group-major
+  uint32_t *lin, *lout;  // nab * per_cu: group level (local exchange), group-major;
+                         // lout stays 0 (the group barrier is all-to-all)
   uint32_t *episodes;
   int nab, per_cu, nxcd;
   int sub_max;           // sub-groups per XCD (at most), as for XcdTree
@@ -677,8 +678,8 @@ struct XcdFlags {
 // members' flags one per lane, and the device master (leader of sub-group 0)
 // gathers the leaders' flags one per lane.  The release is single-hop: the
 // device master writes one release word per XCD (gout[x]) and every WG of
-// that XCD polls its word.  The local-exchange barrier gathers the group's
-// in-flags and releases through its out-flags.  Flags and data are
+// that XCD polls its word.  The local-exchange barrier is all-to-all: each
+// member stores its in-flag and polls the other members' in-flags.  Flags and data are
 // agent-scope accesses, so no level needs a cache fence.
 __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int q,
                                                  uint32_t el, uint32_t eg,
@@ -688,22 +689,18 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
   constexpr long S = kFlagStride;
   wait_mem();
   __syncthreads();
-  if (local_only) {
+  if (local_only) {  // all-to-all within the group: one hop, no master
     if (f.per_cu <= 1) return true;
-    uint32_t *lin = f.lin + (long)grp * f.per_cu * S, *lout = f.lout + (long)grp * f.per_cu * S;
-    if (q != 0) {
-      if (tid == 0) {
-        st_rlx(lin + q * S, el);
-        *s_flag = poll_eq_pipe(lout + q * S, el, poll_q, g);
-      }
-      __syncthreads();
-      return *s_flag != 0;
-    }
+    // Each member stores its own flag and polls the others', one per lane.
+    // Epochs only grow, and a member stores its next one only after the
+    // global barrier in between, which every member reaches after its
+    // polls here, so each poll sees exactly `el` before it moves on.
+    uint32_t *lin = f.lin + (long)grp * f.per_cu * S;
+    if (tid == 0) st_rlx(lin + q * S, el);
     bool ok = true;
-    for (int m = 1 + tid; m < f.per_cu; m += nthr) ok = ok && poll_eq(lin + m * S, el, poll_q, g);
-    if (!__syncthreads_and(ok)) return false;
-    for (int m = 1 + tid; m < f.per_cu; m += nthr) st_rlx(lout + m * S, el);
-    return true;
+    for (int m = tid; m < f.per_cu; m += nthr)
+      if (m != q) ok = ok && poll_eq(lin + m * S, el, poll_q, g);
+    return __syncthreads_and(ok) != 0;
   }
   const int x = grp % f.nxcd;
   const uint32_t *rel = f.gout + x * S;
