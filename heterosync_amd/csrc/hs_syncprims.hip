@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
   const bool lx = (VAR & 1) && a.per_cu > 1;
   const bool all = VAR >= 4;
   const bool tree = VAR <= 1 && a.per_cu > 4;
-  const uint64_t parts = all ? (uint64_t)grid * blockDim.x : (uint64_t)grid;
+  const uint32_t units = all ? (uint32_t)grid * (blockDim.x / 64) : (uint32_t)grid;  // flat arrivals
   const bool wave_lead = (threadIdx.x & 63) == 0;
   const DataPlan dp = data_plan(XCD, lx, VAR == 3, b, grid, nab, a.nxcd);
   const int grp = dp.grp;
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
   }
   __syncthreads();
   if (!s_ok) return;
-  uint32_t epoch = a.e_global0;  // barriers of each kind completed (tree form)
+  uint32_t epoch = a.e_global0;  // barriers of each kind completed (tree and flat forms)
   auto barrier = [&](bool local) -> bool {
     wait_mem();
     __syncthreads();
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
       uint32_t *blk = a.flat + (local ? kBarWords : 0);
       uint32_t *ep = local ? nullptr : a.tb.episodes;
       if (all ? wave_lead : is_master())
-        ok = flat_arrive<XCD>(blk, all ? 64u : 1u, parts, ep, a.bo.poll_q, g);
+        ok = flat_arrive<XCD>(blk, units, epoch, ep, a.bo.poll_q, g);
     }
     return __syncthreads_and(ok) != 0;
   };
@@ -663,12 +663,10 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
     ok = ok && ep == (uint32_t)total_ep;
     const uint32_t te = (uint32_t)total_ep, tl = (uint32_t)(total_ep * local_per_iter);
     if (srb && !srb_tree) {
-      // flat barriers: monotonic 64-bit arrival count, generation = episodes
-      const uint64_t parts = (p >= 41) ? (uint64_t)nw * wg : (uint64_t)nw;
+      // flat barriers: the arrival count wraps to 0 at every episode's last
+      // arrival (atomicInc), and the generation counts the episodes
       auto flat_ok = [&](int k) {
-        uint64_t cnt;
-        std::memcpy(&cnt, &flat[(size_t)k * kBarWords], 8);
-        return cnt == parts * total_ep && flat[(size_t)k * kBarWords + 32] == te;
+        return flat[(size_t)k * kBarWords] == 0 && flat[(size_t)k * kBarWords + 32] == te;
       };
       ok = ok && flat_ok(0);
       if ((p == 40 || p == 42) && per > 1) ok = ok && flat_ok(1);

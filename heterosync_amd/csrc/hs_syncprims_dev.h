@@ -749,32 +749,31 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
 
 // ---------------------------------------------------------------------------
 // Flat (single-level) barrier: the "naive" sense-reversing barriers
-// (hipLocksBarrierAtomicSRB.h:19-64, 227-271).  `weight` is the number of
-// participants one arrival stands for: 1 per WG, or a wave's 64 lanes when
-// every thread joins (NaiveAll: one RMW per wave instead of one per lane).
-// The arrival counter is 64-bit and never reset, so the episode number is
-// the arrival's own return value divided by the participant count -- no
-// separate read of the generation word before arriving and no reset store
-// that a fast WG of the next episode could overtake.  The last arriver
-// publishes generation e+1 with a release store; waiters acquire it.
-//   blk[0..1] arrival count (64-bit), blk[32] generation.
+// (hipLocksBarrierAtomicSRB.h:19-64, 227-271).  `units` is the number of
+// arrivals per episode: one per WG, or one per wave when every thread joins
+// (NaiveAll: one RMW per wave instead of one per lane).  Arrivals are the
+// reference's atomicInc with limit units - 1: the last arrival of an
+// episode wraps the counter back to 0 in the same RMW, so there is no reset
+// store for a fast WG of the next episode to overtake (it arrives only
+// after it has seen the release, and finds the counter at 0).  `epoch` is
+// the number of barriers of this kind completed once this one is: the last
+// arriver publishes it in the generation word, and the others wait for it.
+//   blk[0] arrival count, blk[32] generation.
 // AGENT: the participants' data accesses are agent-scope and drained before
 // they arrive, so the arrival and the generation need no fences.
 template <bool AGENT = false>
-__device__ __forceinline__ bool flat_arrive(uint32_t *blk, uint32_t weight, uint64_t parts,
+__device__ __forceinline__ bool flat_arrive(uint32_t *blk, uint32_t units, uint32_t epoch,
                                             uint32_t *episodes, uint32_t poll_q, Guard &g) {
-  uint64_t *cnt = reinterpret_cast<uint64_t *>(blk);
-  uint32_t *gen = blk + 32;
-  const uint64_t old = AGENT
-      ? __hip_atomic_fetch_add(cnt, (uint64_t)weight, __ATOMIC_RELAXED, HS_AGENT)
-      : __hip_atomic_fetch_add(cnt, (uint64_t)weight, __ATOMIC_ACQ_REL, HS_AGENT);
-  const uint32_t e = (uint32_t)(old / parts);
-  if ((old + weight) % parts == 0) {
-    if (AGENT) st_rlx(gen, e + 1); else st_rel(gen, e + 1);
+  uint32_t *cnt = blk, *gen = blk + 32;
+  const uint32_t old = AGENT
+      ? __builtin_amdgcn_atomic_inc32(cnt, units - 1, __ATOMIC_RELAXED, "agent")
+      : __builtin_amdgcn_atomic_inc32(cnt, units - 1, __ATOMIC_ACQ_REL, "agent");
+  if (old == units - 1) {
+    if (AGENT) st_rlx(gen, epoch); else st_rel(gen, epoch);
     if (episodes) count_rlx(episodes);
     return true;
   }
-  return AGENT ? poll_eq_pipe(gen, e + 1, poll_q, g) : spin_until_eq(gen, e + 1, poll_q, g);
+  return AGENT ? poll_eq_pipe(gen, epoch, poll_q, g) : spin_until_eq(gen, epoch, poll_q, g);
 }
 
 }  // namespace hs
