@@ -42,6 +42,11 @@ __device__ __forceinline__ void st_rel(uint32_t *p, uint32_t v) {
 __device__ __forceinline__ uint32_t add_rlx(uint32_t *p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, HS_AGENT);
 }
+// CUDA/HIP atomicInc: returns the old value and stores old + 1, or 0 once
+// old reaches `limit`, so a counter of n arrivals wraps itself at the nth.
+__device__ __forceinline__ uint32_t inc_wrap_rlx(uint32_t *p, uint32_t limit) {
+  return __builtin_amdgcn_atomic_inc32(p, limit, __ATOMIC_RELAXED, "agent");
+}
 // Bookkeeping count bumped after a release: a non-returning add, so the
 // releaser does not wait on it, and increments from successive releasers
 // commute, so the total is exact once the kernel ends.

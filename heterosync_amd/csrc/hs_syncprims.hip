@@ -671,31 +671,30 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
       ok = ok && flat_ok(0);
       if ((p == 40 || p == 42) && per > 1) ok = ok && flat_ok(1);
       if (p == 38 && per > 1) {  // per-group local barrier
-        // strict: counter reset each episode; fence-free: never reset
+        // counters back at 0 (reset each episode, or wrapped by atomicInc)
         const size_t g0 = o_grp;
-        const uint32_t cnt_want = xcd_mode ? te * (uint32_t)per : 0u;
         for (int gi = 0; gi < nab; ++gi)
-          ok = ok && bar[g0 + kBarWords * gi] == cnt_want && bar[g0 + kBarWords * gi + 32] == te;
+          ok = ok && bar[g0 + kBarWords * gi] == 0 && bar[g0 + kBarWords * gi + 32] == te;
       }
       const uint32_t w[8] = {flat[0], flat[1], flat[32], flat[kBarWords + 32], ep, 0, 0, 0};
       std::memcpy(res->lock_words, w, sizeof(w));
     } else if (p <= 1 || srb) {
       const size_t g0 = o_grp;
       if (xcd_mode) {
-        // counters are never reset: each ends at episodes x its arrivals
+        // atomicInc counters wrap to 0 at each episode's last arrival
         const int nx = std::min(nab, nxcd);
-        ok = ok && bar[0] == te * (uint32_t)n_sub && bar[32] == te;
+        ok = ok && bar[0] == 0 && bar[32] == te;
         for (int x = 0; x < nx; ++x) {
           const int nwx = (nab / nxcd + (x < nab % nxcd ? 1 : 0)) * per;
           for (int k = 0; k < sub_max; ++k) {
             const int ck = std::max(0, std::min(sub, nwx - k * sub));
-            ok = ok && bar[o_sub + (size_t)kBarWords * (x * sub_max + k)] == te * (uint32_t)ck;
+            ok = ok && (ck == 0 || bar[o_sub + (size_t)kBarWords * (x * sub_max + k)] == 0);
           }
           ok = ok && bar[o_rel + (size_t)kBarWords * x] == te;  // release word
         }
         if (per > 1)  // groups: local-exchange arrivals only
           for (int gi = 0; gi < nab; ++gi)
-            ok = ok && bar[g0 + kBarWords * gi] == (lx ? te * (uint32_t)per : 0u) &&
+            ok = ok && bar[g0 + kBarWords * gi] == 0 &&
                  bar[g0 + kBarWords * gi + 32] == (lx ? te : 0u);
       } else {
         ok = ok && bar[0] == 0 && bar[32] == te;

@@ -615,11 +615,12 @@ struct XcdTree {
 };
 
 // Master thread; every wave of the WG has run wait_mem() + __syncthreads().
-// A global arrival climbs two levels of relaxed RMWs on counters that are
-// never reset (the last arriver of a level is the one whose increment
-// completes a multiple of the level's size): the WG's sub-counter (up to
-// sub_wgs WGs of one XCD), then the device counter (one arrival per
-// sub-counter).  The device releaser writes one release word per XCD and
+// A global arrival climbs two levels of relaxed atomicInc RMWs: the WG's
+// sub-counter (up to sub_wgs WGs of one XCD), then the device counter (one
+// arrival per sub-counter).  The last arriver of a level is the one that
+// reads the level's size - 1; its RMW wraps the counter back to 0, so no
+// reset store exists for a WG of the next episode to overtake (it arrives
+// only after the release, which follows every counter's wrap).  The device releaser writes one release word per XCD and
 // every WG of that XCD polls its word, so the release is a single hop.
 // `q` is the WG's index in its group, so (grp / nxcd, q) numbers the WGs of
 // one XCD.  `epoch` is the number of barriers of this kind completed once
@@ -634,7 +635,7 @@ __device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp, int q
   if (local_only) {  // group-level barrier (local exchange, HM:352-353)
     if (per <= 1) return true;
     uint32_t *cnt = t.grp + (long)grp * kBarWords, *gen = cnt + 32;
-    if ((add_rlx(cnt, 1) + 1) % per != 0) return poll_eq_pipe(gen, epoch, poll_q, g);
+    if (inc_wrap_rlx(cnt, per - 1) != per - 1) return poll_eq_pipe(gen, epoch, poll_q, g);
     st_rlx(gen, epoch);
     return true;
   }
@@ -645,8 +646,9 @@ __device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp, int q
   const int sw = t.sub_wgs, k = wx / sw;
   const int ck = nwx - k * sw < sw ? nwx - k * sw : sw;
   uint32_t *sc = t.sub + (long)(x * t.sub_max + k) * kBarWords;
-  if ((add_rlx(sc, 1) + 1) % (uint32_t)ck != 0) return poll_eq_pipe(rel, epoch, poll_q, g);
-  if ((add_rlx(t.glob, 1) + 1) % (uint32_t)t.n_sub != 0) return poll_eq_pipe(rel, epoch, poll_q, g);
+  if (inc_wrap_rlx(sc, (uint32_t)ck - 1) != (uint32_t)ck - 1) return poll_eq_pipe(rel, epoch, poll_q, g);
+  const uint32_t nsub = (uint32_t)t.n_sub;
+  if (inc_wrap_rlx(t.glob, nsub - 1) != nsub - 1) return poll_eq_pipe(rel, epoch, poll_q, g);
   const uint32_t nx = (uint32_t)(t.nab < t.nxcd ? t.nab : t.nxcd);
   for (uint32_t y = 0; y < nx; ++y) st_rlx(t.rel + (long)y * kBarWords, epoch);
   st_rlx(t.glob + 32, epoch);
