@@ -184,6 +184,7 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
   const int b = blockIdx.x, grid = gridDim.x, nab = a.nab;
   const DataPlan dp = data_plan(XCD, LX && a.per_cu > 1, false, b, grid, nab, a.nxcd);
   const int grp = dp.grp, q = dp.q;  // cuID, perCU_WGID (HM:122-132)
+  const XcdSeat seat = (XCD && !FLAG) ? xcd_seat(a.xt, grp, q) : XcdSeat{};
   int cur = dp.id;
   uint32_t eg = a.e_global0, el = a.e_local0;
   if (is_master()) {
@@ -213,7 +214,7 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
         if (XCD) wait_mem();
         __syncthreads();
         if (is_master())
-          s_ok = XCD ? xcd_tree_arrive(a.xt, grp, q, true, a.e_global0 + i + 1, a.bo.poll_q, g)
+          s_ok = XCD ? xcd_tree_arrive(a.xt, seat, grp, true, a.e_global0 + i + 1, a.bo.poll_q, g)
                      : atomic_tree_arrive(a.tb, grp, true, a.bo.poll_q, g);
         __syncthreads();
         ok = s_ok;
@@ -232,7 +233,7 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
       if (XCD) wait_mem();
       __syncthreads();
       if (is_master())
-        s_ok = XCD ? xcd_tree_arrive(a.xt, grp, q, false, a.e_global0 + i + 1, a.bo.poll_q, g)
+        s_ok = XCD ? xcd_tree_arrive(a.xt, seat, grp, false, a.e_global0 + i + 1, a.bo.poll_q, g)
                    : atomic_tree_arrive(a.tb, grp, false, a.bo.poll_q, g);
       __syncthreads();
       ok = s_ok;
@@ -268,6 +269,7 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
   const bool wave_lead = (threadIdx.x & 63) == 0;
   const DataPlan dp = data_plan(XCD, lx, VAR == 3, b, grid, nab, a.nxcd);
   const int grp = dp.grp;
+  const XcdSeat seat = (XCD && tree) ? xcd_seat(a.xt, grp, dp.q) : XcdSeat{};
   int cur = dp.id;
   if (is_master()) {
     s_ok = !g.aborted();
@@ -291,7 +293,7 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
     bool ok = true;
     if ((local && VAR == 1) || (!local && tree)) {
       if (is_master())
-        ok = XCD ? xcd_tree_arrive(a.xt, grp, dp.q, local, epoch, a.bo.poll_q, g)
+        ok = XCD ? xcd_tree_arrive(a.xt, seat, grp, local, epoch, a.bo.poll_q, g)
                  : atomic_tree_arrive(a.tb, grp, local, a.bo.poll_q, g);
     } else {
       uint32_t *blk = a.flat + (local ? kBarWords : 0);

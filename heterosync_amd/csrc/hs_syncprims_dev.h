@@ -614,6 +614,24 @@ struct XcdTree {
   int sub_wgs;         // WGs per sub-counter
 };
 
+// A WG's fixed place in the XCD-aware tree, computed once per kernel so
+// that the divisions locating it stay off every arrival's path.
+struct XcdSeat {
+  uint32_t *sc;       // its sub-counter
+  uint32_t *rel;      // its XCD's release word
+  uint32_t sub_last;  // arrivals at that sub-counter - 1
+};
+__device__ __forceinline__ XcdSeat xcd_seat(const XcdTree &t, int grp, int q) {
+  const int per = t.per_cu;
+  const int x = grp % t.nxcd;
+  const int nwx = groups_on_xcd(t.nab, t.nxcd, x) * per;  // WGs on this XCD
+  const int wx = (grp / t.nxcd) * per + q;
+  const int sw = t.sub_wgs, k = wx / sw;
+  const int ck = nwx - k * sw < sw ? nwx - k * sw : sw;
+  return XcdSeat{t.sub + (long)(x * t.sub_max + k) * kBarWords, t.rel + (long)x * kBarWords,
+                 (uint32_t)ck - 1};
+}
+
 // Master thread; every wave of the WG has run wait_mem() + __syncthreads().
 // A global arrival climbs two levels of relaxed atomicInc RMWs: the WG's
 // sub-counter (up to sub_wgs WGs of one XCD), then the device counter (one
@@ -622,13 +640,13 @@ struct XcdTree {
 // reset store exists for a WG of the next episode to overtake (it arrives
 // only after the release, which follows every counter's wrap).  The device releaser writes one release word per XCD and
 // every WG of that XCD polls its word, so the release is a single hop.
-// `q` is the WG's index in its group, so (grp / nxcd, q) numbers the WGs of
-// one XCD.  `epoch` is the number of barriers of this kind completed once
+// `seat` locates the WG's sub-counter and release word (xcd_seat).
+// `epoch` is the number of barriers of this kind completed once
 // this one is: the value the release word (global) or the group's
 // generation (local_only) must reach, so nothing is read before arriving.
 // The WGs' data accesses are agent-scope where they leave the XCD and
 // complete before they arrive, so no level needs a cache fence.
-__device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp, int q,
+__device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, const XcdSeat &seat, int grp,
                                                 bool local_only, uint32_t epoch,
                                                 uint32_t poll_q, Guard &g) {
   const uint32_t per = (uint32_t)t.per_cu;
@@ -639,14 +657,8 @@ __device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, int grp, int q
     st_rlx(gen, epoch);
     return true;
   }
-  const int x = grp % t.nxcd;
-  uint32_t *rel = t.rel + (long)x * kBarWords;
-  const int nwx = groups_on_xcd(t.nab, t.nxcd, x) * (int)per;  // WGs on this XCD
-  const int wx = (grp / t.nxcd) * (int)per + q;
-  const int sw = t.sub_wgs, k = wx / sw;
-  const int ck = nwx - k * sw < sw ? nwx - k * sw : sw;
-  uint32_t *sc = t.sub + (long)(x * t.sub_max + k) * kBarWords;
-  if (inc_wrap_rlx(sc, (uint32_t)ck - 1) != (uint32_t)ck - 1) return poll_eq_pipe(rel, epoch, poll_q, g);
+  uint32_t *rel = seat.rel;
+  if (inc_wrap_rlx(seat.sc, seat.sub_last) != seat.sub_last) return poll_eq_pipe(rel, epoch, poll_q, g);
   const uint32_t nsub = (uint32_t)t.n_sub;
   if (inc_wrap_rlx(t.glob, nsub - 1) != nsub - 1) return poll_eq_pipe(rel, epoch, poll_q, g);
   const uint32_t nx = (uint32_t)(t.nab < t.nxcd ? t.nab : t.nxcd);
