@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 # a 256 MiB copy kernel (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the command
 # lines and raw figures are in profiles/k_barrier_atomicTreeBarrUniq_1024x256.txt.
 # Key: (prim, sync scope, WG size, numLdSt, numCSIters, num WGs).
-MEASURED_TRAFFIC = {("atomicTreeBarrUniq", 0, 256, 10, 1000, 1024): 403555667}
+MEASURED_TRAFFIC = {("atomicTreeBarrUniq", 0, 256, 10, 1000, 1024): 405270138}
 METRIC = ("acquire+release/s & barrier episodes/s at 1 GPU, batched vs plain; "
           "UTS nodes/s, p99")
 
