@@ -24,6 +24,8 @@ def seq_expected(oracle, nw, gs, period, iters, n):
     (512, 128, 4, 16, 10),     # the reference's run file: ./seqlocks 512 128 4 {0,1}
     (256, 64, 16, 65, 200),
     (1024, 256, 16, 65, 50),   # full-occupancy shape, 1 writer WG per 64 readers
+    (2048, 64, 8, 33, 100),    # 8 WGs per CU, more writers
+    (128, 1024, 2, 16, 20),    # 1024-thread WGs
 ])
 def test_seqlock(dev, oracle, variant, nw, wg, gs, period, iters):
     seq, out, torn, r = dev.seqlock(nw, wg, gs, period, iters, variant)

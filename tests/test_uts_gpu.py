@@ -38,10 +38,19 @@ def geo_count(oracle, seed, b0, d):
 
 
 @pytest.mark.parametrize("seed,d,blocks", [(0, 1, 0), (1, 4, 0), (19, 6, 0), (7, 8, 64),
-                                           (123, 9, 0), (5, 0, 0)])
+                                           (123, 9, 0), (5, 0, 0), (42, 10, 0)])
 def test_uts_geo_trees(dev, oracle, seed, d, blocks):
     want = geo_count(oracle, seed, 4.0, d)
     r = dev.uts_geo(seed, 4.0, d, blocks)
+    assert (r.nodes, r.leaves, r.max_depth) == want
+
+
+@pytest.mark.parametrize("seed,b0,d", [(11, 2.0, 16), (12, 8.0, 6), (13, 3.5, 9)])
+def test_uts_geo_other_branching(dev, oracle, seed, b0, d):
+    """Other expected branching factors: the child-count threshold table is
+    built per b0."""
+    want = geo_count(oracle, seed, b0, d)
+    r = dev.uts_geo(seed, b0, d, 0)
     assert (r.nodes, r.leaves, r.max_depth) == want
 
 
