@@ -29,6 +29,22 @@ def test_batch_mix_parity(dev, ohelp, use_graph):
     assert r.acquires == int(entries.sum())
 
 
+@pytest.mark.parametrize("seed", [17, 29, 41])
+@pytest.mark.parametrize("workers", [256, 2048])
+def test_batch_mix_seeds_and_workers(dev, ohelp, seed, workers):
+    """Other mixes, with the virtual WGs spread over few workers (many per
+    worker) or many (a few each)."""
+    mix = make_mix(seed=seed, n=32, iters_cap=15)
+    st = storages_for(dev, mix)
+    entries, r = dev.batch(mix, st, reps=2, use_graph=1, persistent_wgs=workers)
+    assert r.all_ok == 1
+    ncu = dev.info.num_cu
+    for (p, nw, it, L), s, e in zip(mix, st, entries):
+        assert np.array_equal(s, ohelp.golden(p, L, nw, it, 2, 64, ncu, 0)), (p, nw, it, L)
+        w, rd, _ = ohelp.counts(p, nw, it, 2, ncu)
+        assert e == w + rd
+
+
 def test_batch_alone_equals_plain(dev):
     mix = make_mix(seed=11, n=8, iters_cap=10)
     st = storages_for(dev, mix)
