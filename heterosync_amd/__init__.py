@@ -27,7 +27,8 @@ ERRORS = {
 # Every symbol include/heterosync.h declares.
 EXPORTED = [
     "hs_strerror", "hs_open", "hs_close", "hs_device_info", "hs_prim_id",
-    "hs_prim_name", "hs_prim_supported", "hs_storage_words", "hs_syncprim_run",
+    "hs_prim_name", "hs_prim_supported", "hs_storage_words", "hs_syncprim_expected",
+    "hs_syncprim_run",
     "hs_seqlock_run", "hs_flags_run", "hs_refcounter_run", "hs_splitcounter_run",
     "hs_uts_ref_run", "hs_uts_geo_run", "hs_batch_run", "hs_batch_autotune",
     "hs_uts_serve",
@@ -105,7 +106,9 @@ class ServeCfg(ctypes.Structure):
 
 class ServeSummary(ctypes.Structure):
     _fields_ = [("wall_s", ctypes.c_double), ("total_nodes", ctypes.c_uint64),
-                ("p50_s", ctypes.c_double), ("p99_s", ctypes.c_double)]
+                ("p50_s", ctypes.c_double), ("p99_s", ctypes.c_double),
+                ("cut_batches", ctypes.c_uint32), ("deque_high_water", ctypes.c_uint32),
+                ("deque_soft", ctypes.c_uint32)]
 
 
 _LIB = None
@@ -133,6 +136,8 @@ def _lib():
         lib.hs_prim_supported.argtypes = [ctypes.c_int]
         lib.hs_storage_words.restype = ctypes.c_long
         lib.hs_storage_words.argtypes = [ctypes.c_int] * 5
+        lib.hs_syncprim_expected.argtypes = [ctypes.c_int] * 8 + [P(ctypes.c_int32),
+                                                                  ctypes.c_long]
         lib.hs_syncprim_run.argtypes = [ctypes.c_void_p, P(SyncPrimCfg), P(ctypes.c_int32),
                                         ctypes.c_long, P(SyncPrimResult)]
         lib.hs_seqlock_run.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [
@@ -143,7 +148,7 @@ def _lib():
             P(ctypes.c_uint32), P(RelaxedResult)]
         lib.hs_refcounter_run.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [
             P(ctypes.c_uint32)] * 4 + [P(RelaxedResult)]
-        lib.hs_splitcounter_run.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 3 + [
+        lib.hs_splitcounter_run.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4 + [
             P(ctypes.c_uint32), P(ctypes.c_uint32), P(RelaxedResult)]
         lib.hs_uts_ref_run.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_uint32),
                                        P(ctypes.c_int32), P(ctypes.c_int32), ctypes.c_int,
@@ -186,6 +191,20 @@ def prim_name(pid: int):
 
 def storage_words(prim: int, num_wgs: int, num_cu: int, wg_size: int, num_ldst: int) -> int:
     return _lib().hs_storage_words(prim, num_wgs, num_cu, wg_size, num_ldst)
+
+
+def expected_storage(prim, num_ldst, num_wgs, num_cs_iters, reps, wg_size, num_cu, mode):
+    """The storage a correct run leaves (hs_syncprim_expected: the reference
+    binary's own closed-form self-check).  Host code; needs no device."""
+    pid = prim if isinstance(prim, int) else prim_id(prim)
+    n = storage_words(pid, num_wgs, num_cu, wg_size, num_ldst)
+    if n < 0:
+        raise HSError(n, "hs_storage_words")
+    out = np.empty(n, np.int32)
+    _check(_lib().hs_syncprim_expected(pid, num_ldst, num_wgs, num_cs_iters, reps, wg_size,
+                                       num_cu, mode, _ptr(out, ctypes.c_int32), n),
+           "hs_syncprim_expected")
+    return out
 
 
 @dataclass
@@ -276,12 +295,12 @@ class Device:
                "hs_refcounter_run")
         return c0, c1, d0, d1, res
 
-    def splitcounter(self, num_tbs, tb_size, iters):
+    def splitcounter(self, num_tbs, tb_size, iters, read_every=0):
         nthr = num_tbs * tb_size
         counters = np.zeros(nthr // 2, np.uint32)
         out = np.zeros(nthr, np.uint32)
         res = RelaxedResult()
-        _check(_lib().hs_splitcounter_run(self._h, num_tbs, tb_size, iters,
+        _check(_lib().hs_splitcounter_run(self._h, num_tbs, tb_size, iters, read_every,
                                           _ptr(counters, ctypes.c_uint32),
                                           _ptr(out, ctypes.c_uint32), ctypes.byref(res)),
                "hs_splitcounter_run")
@@ -315,15 +334,25 @@ class Device:
         return list(resp), summ
 
     # ---- batched ---------------------------------------------------------
-    def batch_autotune(self, instances, reps=10, use_graph=1, timeout_s=20.0):
-        """Returns the tuned (persistent_wgs, poll_sleep, backoff_max, best_ms)."""
+    def batch_autotune(self, instances, reps=10, mode=0, use_graph=1, timeout_s=20.0):
+        """Returns the tuned knobs as a dict: wg_size, persistent_wgs,
+        poll_sleep, backoff_min, backoff_max, and best_ms (the winning
+        calibration launch).  batch(**{k: v for k in TUNED}) runs them."""
         n = len(instances)
         inst = (BatchInstance * n)(*[BatchInstance(*i) for i in instances])
-        cfg = BatchCfg(64, reps, 0, 0, 0, 0, use_graph, 0, timeout_s)
+        cfg = BatchCfg(64, reps, mode, 0, 0, 0, use_graph, 0, timeout_s)
         best = ctypes.c_double()
         _check(_lib().hs_batch_autotune(self._h, inst, n, ctypes.byref(cfg), ctypes.byref(best)),
                "hs_batch_autotune")
-        return cfg.persistent_wgs, cfg.poll_sleep, cfg.backoff_max, best.value
+        return {"wg_size": cfg.wg_size, "persistent_wgs": cfg.persistent_wgs,
+                "poll_sleep": cfg.poll_sleep, "backoff_min": cfg.backoff_min,
+                "backoff_max": cfg.backoff_max, "best_ms": best.value}
+
+    @staticmethod
+    def tuned_kwargs(t):
+        """batch() keyword arguments for a batch_autotune() result."""
+        return dict(wg_size=t["wg_size"], persistent_wgs=t["persistent_wgs"],
+                    poll_sleep=t["poll_sleep"], backoff=(t["backoff_min"], t["backoff_max"]))
 
     def batch(self, instances, storages, wg_size=64, reps=10, mode=0, backoff=(0, 0),
               poll_sleep=0, use_graph=1, persistent_wgs=0, timeout_s=20.0):

@@ -10,10 +10,8 @@
 // scripts keep working: --wg=N --mode=copy|inc --reps=N --backoff=MIN:MAX
 // --poll=N --timeout=S --json.
 //
-// The self-check is the closed form of the reference's host golden
-// (main.hip.cpp:2524-2682): every chain of L+1 16-word lines that k passes
-// touched ends as "line j = original line max(0, j-k)" (copy) or
-// "original + k on lines 1..L" (inc).
+// The self-check is the library's hs_syncprim_expected: the closed form of
+// the reference's host golden (main.hip.cpp:2524-2682).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -72,42 +70,6 @@ std::string banner(int p, int iters) {
     std::snprintf(buf, sizeof buf, "%s_%03d_%03d", kind, shown, iters);
   }
   return buf;
-}
-
-// Passes over each WG-sized region implied by the kernels' region walk.
-bool region_passes(int p, int nw, int iters, int reps, int ncu, std::vector<long> &k) {
-  const long ir = (long)iters * reps;
-  const int nab = nw < ncu ? nw : ncu;
-  const int per = (nw + nab - 1) / nab;
-  long writers = 0;  // semaphores: one writer per group (main.hip.cpp:883)
-  for (int b = 0; b < nw; ++b) writers += (b / nab == (b % nab) % per);
-  if (p < 4 || (p >= 37 && p <= 42)) {  // barriers: every region, twice with local exchange
-    const bool lx = p == 1 || p == 3 || p == 38 || p == 40 || p == 42;
-    k.assign(nw, ir * ((lx && per > 1) ? 2 : 1));
-  } else if (p <= 7) {
-    k.assign(1, (long)nw * ir);
-  } else if (p <= 19) {  // global semaphores: regions 0..perCU-1
-    k.assign(per, writers * ir);
-  } else if (p >= 32 && p <= 35) {  // global EBO priority kernel over num_wgs regions
-    k.assign(nw, 0);
-    for (int r = 0; r < per && r < nw; ++r) k[r] = writers * ir;
-  } else if (p >= 20 && p <= 23) {
-    k.assign(nab, 0);
-    for (int b = 0; b < nw; ++b) k[b % ncu] += ir;
-  } else if (p >= 24 && p <= 31) {
-    k.assign(nw, 0);
-    for (int b = 0; b < nw; ++b) {
-      if (b / nab != (b % nab) % per) continue;
-      int cur = b;
-      for (int j = 0; j < per; ++j) {
-        k[cur] += ir;
-        cur = (cur + nab) % nw;
-      }
-    }
-  } else {
-    return false;
-  }
-  return true;
 }
 
 }  // namespace
@@ -209,25 +171,19 @@ int main(int argc, char **argv) {
   std::printf("elapsed time: %f ms, average: %f ms\n", res.roi_ms, res.roi_ms / reps);
   std::fflush(stdout);
 
-  std::vector<long> k;
-  if (!region_passes(p, nw, iters, reps, ncu, k) ||
-      (long)k.size() * uniq_per_wg != n) {  // every storage region has a pass count
+  std::vector<int32_t> golden(n);
+  if (hs_syncprim_expected(p, L, nw, iters, reps, wg, ncu, mode, golden.data(), n) != HS_OK) {
     std::fprintf(stderr, "ERROR: no self-check for %s\n", prim_str);
     hs_close(ctx);
     return -1;
   }
-  const long chain = 16L * (L + 1);
   unsigned errors = 0;
   std::printf("Comparing GPU results to golden results:\n");
   for (long i = 0; i < n; ++i) {
-    const long kk = k[i / uniq_per_wg];
-    const long c0 = (i / chain) * chain, j = (i - c0) / 16, w = (i - c0) % 16;
-    const long golden = mode == 0 ? c0 + (j - kk > 0 ? j - kk : 0) * 16 + w
-                                  : i + (j >= 1 ? kk : 0);
-    if (storage[i] != golden) {
+    if (storage[i] != golden[i]) {
       if (errors < 16)
-        std::fprintf(stderr, "\tERROR: storage[%ld] = %d, golden[%ld] = %ld\n", i,
-                     storage[i], i, golden);
+        std::fprintf(stderr, "\tERROR: storage[%ld] = %d, golden[%ld] = %d\n", i,
+                     storage[i], i, golden[i]);
       ++errors;
     }
   }

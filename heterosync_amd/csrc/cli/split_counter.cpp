@@ -1,7 +1,8 @@
 // split_counter.cpp -- drop-in for `./splitCounter <numTBs> <tbSize>`
 // (cuda/relaxedAtomics/uvm/splitCounter/splitCounter.cu).  Checks every
 // counter == iters*INC_VAL (splitCounter.cu:52-60) and that every reader's
-// approximate total lies in [0, total].  Option: --iters=N (reference 1).
+// approximate total lies in [0, total].  Options: --iters=N (reference 1),
+// --read-every=N (writer iterations between a WG's reads; 0 = min(64, N)).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -11,20 +12,22 @@
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "./splitCounter <numTBs> <tbSize> [--iters=N]\n");
+    std::fprintf(stderr, "./splitCounter <numTBs> <tbSize> [--iters=N] [--read-every=N]\n");
     return -1;
   }
   const int ntb = std::atoi(argv[1]), tb = std::atoi(argv[2]);
-  int iters = 1;
-  for (int i = 3; i < argc; ++i)
+  int iters = 1, read_every = 0;
+  for (int i = 3; i < argc; ++i) {
     if (!std::strncmp(argv[i], "--iters=", 8)) iters = std::atoi(argv[i] + 8);
+    else if (!std::strncmp(argv[i], "--read-every=", 13)) read_every = std::atoi(argv[i] + 13);
+  }
   hs_ctx *ctx;
   if (hs_open(0, &ctx)) { std::fprintf(stderr, "no HIP device\n"); return -1; }
   const size_t nthr = (size_t)ntb * tb;
   std::vector<uint32_t> cnt(nthr / 2), out(nthr);
   hs_relaxed_result r;
   std::printf("Launching kernel - 1 runs with %d TBs and %d threads/TB\n", ntb, tb);
-  int rc = hs_splitcounter_run(ctx, ntb, tb, iters, cnt.data(), out.data(), &r);
+  int rc = hs_splitcounter_run(ctx, ntb, tb, iters, read_every, cnt.data(), out.data(), &r);
   if (rc) { std::fprintf(stderr, "ERROR hs_splitcounter_run - %s\n", hs_strerror(rc)); return -1; }
   bool pass = true;
   for (size_t i = 0; i < cnt.size(); ++i)

@@ -58,11 +58,76 @@ __device__ __forceinline__ int find_inst(const BInst *in, int n, int v) {
   return lo;
 }
 
-__global__ void __launch_bounds__(64) k_batch(BArgs a) {
+// Critical section of one 64-thread virtual WG run by a worker of 64 * S
+// lanes.  Lane t works on column t % 64 of the virtual WG (the reference's
+// thread t % 64, main.hip.cpp:127-128) and on slice t / 64 of its chain's
+// L lines.  S = 1: the per-lane top-down chunk walk.  S > 1, inc mode: the
+// slices are disjoint line ranges, each incremented in place.  S > 1, copy
+// mode: the chain moves band by band, top band first, each band 16 * S
+// source lines (16 per slice); every load of a band completes before any of
+// its stores (WG barrier), and a band's stores (lines lo+1..hi) never touch
+// the lines the band below still has to read (lines under lo).
+template <int MODE, int S>
+__device__ __forceinline__ void vwg_access(int32_t *s, long tid0, int L) {
+  const int col = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  const long tid = tid0 + col;
+  const long base = (tid / kWordsPerLine) * (L + 1);
+  const int off = col % kWordsPerLine;
+  if (S == 1) {
+    access_data<MODE, true>(s, base, off, L);
+    return;
+  }
+  if (MODE == 1) {
+    const int per = (L + S - 1) / S, lo = sl * per, hi = lo + per < L ? lo + per : L;
+    if (hi > lo) access_data<1, true>(s, base + lo, off, hi - lo);
+    return;
+  }
+  int32_t *c = s + base * kWordsPerLine + off;
+  for (int top = L; top > 0; top -= 16 * S) {
+    const int blo = top > 16 * S ? top - 16 * S : 0;
+    const int lo = blo + 16 * sl;
+    const int n = top - lo < 16 ? top - lo : 16;  // <= 0: no lines in this slice
+    int32_t v[16];
+    if (n == 16) chunk_ld<0, 16, true>(c, lo, v);
+    else if (n > 0) chunk_tail<0, true, false>(c + (long)lo * kWordsPerLine, n, v);
+    wait_mem();
+    __syncthreads();
+    if (n == 16) chunk_st<0, 16, true>(c, lo, v);
+    else if (n > 0) chunk_tail<0, true, true>(c + (long)lo * kWordsPerLine, n, v);
+  }
+}
+
+// Semaphore reader body (accessData_semRd, main.hip.cpp:89-100) over the
+// same column / slice split: the lane's lines, loaded eight at a time, summed
+// into LDS so the loads are not dead.
+template <int S>
+__device__ __forceinline__ void vwg_read(const int32_t *s, long tid0, int L,
+                                         volatile int32_t *sink) {
+  const int col = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  const long tid = tid0 + col;
+  const int32_t *c = s + (tid / kWordsPerLine) * (L + 1) * kWordsPerLine + col % kWordsPerLine;
+  const int per = (L + S - 1) / S, lo = sl * per, hi = lo + per < L ? lo + per : L;
+  int32_t acc = 0;
+  for (int n0 = lo; n0 < hi; n0 += 8) {
+    int32_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = n0 + u < hi ? ld_data<true>(c + (n0 + u) * kWordsPerLine) : 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
+  sink[threadIdx.x] += acc;
+}
+
+// MODE 0: the reference's copy chain; MODE 1: in-place increment of lines
+// 1..L, so a lost, doubled or overlapping critical section changes the
+// final storage (with a copy chain every pass past the L-th leaves the same
+// array).  S: waves per worker WG; all of them share each critical section.
+template <int MODE, int S>
+__global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
   __shared__ int s_inst[kMaxVpw], s_b[kMaxVpw], s_iter[kMaxVpw], s_phase[kMaxVpw];
   __shared__ uint32_t s_ticket[kMaxVpw], s_skip[kMaxVpw], s_bo[kMaxVpw];
   __shared__ int s_remaining, s_enter, s_ok;
-  __shared__ int32_t sink[64];
+  __shared__ int32_t sink[64 * S];
   const int lane = threadIdx.x, P = gridDim.x, w = blockIdx.x;
   Guard g;
   g.init(a.ctl, a.ticks);
@@ -139,11 +204,11 @@ __global__ void __launch_bounds__(64) k_batch(BArgs a) {
       if (!s_enter) continue;
       // ---- critical section of virtual WG b of instance I --------------
       if (!sem) {
-        wg_access<0, true>(I.storage, 0, I.L);
+        vwg_access<MODE, S>(I.storage, 0, I.L);
       } else if (writer) {
-        for (int k = 0; k < I.per; ++k) wg_access<0, true>(I.storage, (long)k * 64, I.L);
+        for (int k = 0; k < I.per; ++k) vwg_access<MODE, S>(I.storage, (long)k * 64, I.L);
       } else {
-        wg_read<true>(I.storage, (long)pw * 64, I.L, sink);
+        vwg_read<S>(I.storage, (long)pw * 64, I.L, sink);
       }
       wait_mem();  // every lane's CS accesses complete before the release
       __syncthreads();
@@ -257,10 +322,30 @@ int batch_prepare(hs_ctx *c, const hs_batch_instance *in, int n, int reps, Batch
   return HS_OK;
 }
 
+using BatchFn = void (*)(BArgs);
+
+// Worker width in waves: 64, 128 or 256 lanes per worker WG (0 = 64).
+int worker_waves(const hs_batch_cfg *cfg) {
+  const int w = cfg->wg_size > 0 ? cfg->wg_size : 64;
+  return w == 64 ? 1 : w == 128 ? 2 : w == 256 ? 4 : 0;
+}
+
+BatchFn pick_batch(int mode, int waves) {
+  switch (waves) {
+    case 1: return mode ? k_batch<1, 1> : k_batch<0, 1>;
+    case 2: return mode ? k_batch<1, 2> : k_batch<0, 2>;
+    case 4: return mode ? k_batch<1, 4> : k_batch<0, 4>;
+    default: return nullptr;
+  }
+}
+
 // Runs `reps` persistent launches (optionally as one captured graph) and
 // returns the ROI in ms.
 int batch_launch(hs_ctx *c, Batch &B, int n, const hs_batch_cfg *cfg, int reps, int workers,
                  double *roi_ms, double *kern_ms) {
+  const int S = worker_waves(cfg);
+  const BatchFn kfn = pick_batch(cfg->mode, S);
+  if (!kfn) return HS_EINVAL;
   BArgs a;
   a.inst = B.d_inst;
   a.ninst = n;
@@ -287,7 +372,7 @@ int batch_launch(hs_ctx *c, Batch &B, int n, const hs_batch_cfg *cfg, int reps, 
   if (cfg->use_graph) {
     HS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     for (int r = 0; r < reps; ++r)
-      hipLaunchKernelGGL(k_batch, dim3(P), dim3(64), 0, c->stream, a);
+      hipLaunchKernelGGL(kfn, dim3(P), dim3(64 * S), 0, c->stream, a);
     HS_HIP(hipStreamEndCapture(c->stream, &gr.g));
     HS_HIP(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0));
   }
@@ -297,7 +382,7 @@ int batch_launch(hs_ctx *c, Batch &B, int n, const hs_batch_cfg *cfg, int reps, 
     HS_HIP(hipGraphLaunch(gr.x, c->stream));
   } else {
     for (int r = 0; r < reps; ++r) {
-      hipLaunchKernelGGL(k_batch, dim3(P), dim3(64), 0, c->stream, a);
+      hipLaunchKernelGGL(kfn, dim3(P), dim3(64 * S), 0, c->stream, a);
       HS_HIP(hipGetLastError());
     }
   }
@@ -310,11 +395,22 @@ int batch_launch(hs_ctx *c, Batch &B, int n, const hs_batch_cfg *cfg, int reps, 
   return HS_OK;
 }
 
+// Worker WGs of the persistent grid: cfg->persistent_wgs, or 8 waves per CU
+// by default, doubled while a worker would carry more than kMaxVpw virtual
+// WGs.
 int default_workers(const hs_ctx *c, const hs_batch_cfg *cfg, int nv) {
-  const int per_cu = cfg->persistent_wgs > 0 ? 0 : 8;
-  int P = cfg->persistent_wgs > 0 ? cfg->persistent_wgs : per_cu * c->num_cu;
+  const int S = std::max(1, worker_waves(cfg));
+  int P = cfg->persistent_wgs > 0 ? cfg->persistent_wgs : std::max(1, 8 * c->num_cu / S);
   while ((nv + P - 1) / P > kMaxVpw) P *= 2;
   return P;
+}
+
+// Refuses a grid whose workers cannot all be resident at once.
+int check_workers(hs_ctx *c, const hs_batch_cfg *cfg, int P, int nv) {
+  const int S = worker_waves(cfg);
+  int fit = 0, rc;
+  if ((rc = coresident_wgs(c, (const void *)pick_batch(cfg->mode, S), 64 * S, &fit))) return rc;
+  return fit < std::min(P, nv) ? HS_EOCCUPANCY : HS_OK;
 }
 
 }  // namespace
@@ -323,8 +419,8 @@ extern "C" int hs_batch_run(hs_ctx *c, const hs_batch_instance *in, int n,
                             const hs_batch_cfg *cfg, int32_t *const *storages, int64_t *entries,
                             hs_batch_result *res) {
   if (!c || !in || n < 1 || !cfg || !storages || !entries || !res) return HS_EINVAL;
-  if (cfg->wg_size != 0 && cfg->wg_size != 64) return HS_EINVAL;  // wave64 virtual WGs
-  if (cfg->mode != 0) return HS_EINVAL;
+  if (worker_waves(cfg) == 0) return HS_EINVAL;  // workers of 64, 128 or 256 lanes
+  if (cfg->mode != 0 && cfg->mode != 1) return HS_EINVAL;
   std::memset(res, 0, sizeof *res);
   const int reps = cfg->reps > 0 ? cfg->reps : 10;
   HS_HIP(hipSetDevice(c->device));
@@ -332,9 +428,7 @@ extern "C" int hs_batch_run(hs_ctx *c, const hs_batch_instance *in, int n,
   int rc = batch_prepare(c, in, n, reps, B, storages);
   if (rc) return rc;
   const int P = default_workers(c, cfg, B.nv);
-  int fit = 0;
-  if ((rc = coresident_wgs(c, (const void *)k_batch, 64, &fit))) return rc;
-  if (fit < std::min(P, B.nv)) return HS_EOCCUPANCY;
+  if ((rc = check_workers(c, cfg, P, B.nv))) return rc;
   double roi = 0, kern = 0;
   if ((rc = batch_launch(c, B, n, cfg, reps, P, &roi, &kern))) return rc;
   res->roi_ms = roi;
@@ -364,37 +458,76 @@ extern "C" int hs_batch_run(hs_ctx *c, const hs_batch_instance *in, int n,
 extern "C" int hs_batch_autotune(hs_ctx *c, const hs_batch_instance *in, int n,
                                  hs_batch_cfg *cfg, double *best_ms) {
   if (!c || !in || n < 1 || !cfg) return HS_EINVAL;
+  if (cfg->mode != 0 && cfg->mode != 1) return HS_EINVAL;
   HS_HIP(hipSetDevice(c->device));
   // calibration copy: iterations capped so each trial takes milliseconds
   std::vector<hs_batch_instance> cal(in, in + n);
   for (auto &x : cal) x.num_cs_iters = std::min(x.num_cs_iters, 20);
-  static const int kWorkersPerCu[] = {2, 4, 8, 16};
-  static const int kPoll[] = {0, 1, 4};
-  static const int kBackoffMax[] = {63, 255, 1023};
   double best = 1e30;
   hs_batch_cfg best_cfg = *cfg;
-  for (int wpc : kWorkersPerCu)
+  best_cfg.wg_size = 64;
+  best_cfg.backoff_min = 1;
+  int best_wpc = 8;
+  // One calibration launch of t; *ms < 0 when its workers cannot all be resident.
+  auto trial = [&](const hs_batch_cfg &t, double *ms) -> int {
+    Batch B;
+    int rc = batch_prepare(c, cal.data(), n, 1, B, nullptr);
+    if (rc) return rc;
+    const int P = default_workers(c, &t, B.nv);
+    rc = check_workers(c, &t, P, B.nv);
+    if (rc == HS_EOCCUPANCY) {
+      *ms = -1;
+      return HS_OK;
+    }
+    if (rc) return rc;
+    if ((rc = batch_launch(c, B, n, &t, 1, P, ms, nullptr))) return rc;
+    uint32_t ab = 0;
+    HS_HIP(hipMemcpy(&ab, B.d_ctl, 4, hipMemcpyDeviceToHost));
+    return ab ? HS_ETIMEOUT : HS_OK;
+  };
+  // stage 1, 64-lane workers starting backoff at 1 quantum (the reference's
+  // NUM_WIS_PER_WG and backoff start, main.hip.cpp:9, hipLocksMutexEBO.h:26):
+  // waves per CU x poll interval x EBO cap
+  static const int kWavesPerCu[] = {2, 4, 8, 16};
+  static const int kPoll[] = {0, 1, 4};
+  static const int kBackoffMax[] = {63, 255, 1023};
+  for (int wpc : kWavesPerCu)
     for (int poll : kPoll)
       for (int bmax : kBackoffMax) {
-        hs_batch_cfg t = *cfg;
+        hs_batch_cfg t = best_cfg;
         t.persistent_wgs = wpc * c->num_cu;
         t.poll_sleep = poll == 0 ? -1 : poll;
         t.backoff_max = bmax;
         t.reps = 1;
-        Batch B;
-        int rc = batch_prepare(c, cal.data(), n, 1, B, nullptr);
+        double ms = 0;
+        const int rc = trial(t, &ms);
         if (rc) return rc;
-        const int P = default_workers(c, &t, B.nv);
-        double roi = 0;
-        if ((rc = batch_launch(c, B, n, &t, 1, P, &roi, nullptr))) return rc;
-        uint32_t ab = 0;
-        HS_HIP(hipMemcpy(&ab, B.d_ctl, 4, hipMemcpyDeviceToHost));
-        if (ab) return HS_ETIMEOUT;
-        if (roi < best) {
-          best = roi;
+        if (ms >= 0 && ms < best) {
+          best = ms;
           best_cfg = t;
+          best_wpc = wpc;
         }
       }
+  // stage 2, at the same waves per CU: worker width (lanes sharing each
+  // critical section) x backoff start I_min
+  static const int kWidth[] = {64, 128, 256};
+  static const int kBackoffMin[] = {1, 4, 16};
+  const hs_batch_cfg base = best_cfg;
+  for (int wd : kWidth)
+    for (int bmin : kBackoffMin) {
+      if (wd == 64 && bmin == 1) continue;  // stage 1's winner
+      hs_batch_cfg t = base;
+      t.wg_size = wd;
+      t.persistent_wgs = std::max(1, best_wpc * c->num_cu / (wd / 64));
+      t.backoff_min = bmin;
+      double ms = 0;
+      const int rc = trial(t, &ms);
+      if (rc) return rc;
+      if (ms >= 0 && ms < best) {
+        best = ms;
+        best_cfg = t;
+      }
+    }
   best_cfg.reps = cfg->reps;
   *cfg = best_cfg;
   if (best_ms) *best_ms = best;

@@ -175,4 +175,58 @@ long hs_storage_words(int p, int num_wgs, int num_cu, int wg, int L) {
   return mult * (long)wg * (long)(L + 1);
 }
 
+// The reference binary's own self-check (main.hip.cpp:2524-2702) in closed
+// form.  The kernels' region walk fixes how many critical-section passes k
+// touch each WG-sized region; a chain of L+1 16-word lines that k passes
+// touched ends as "line j = original line max(0, j-k)" (copy) or
+// "original + k on lines 1..L" (inc).
+int hs_syncprim_expected(int p, int L, int nw, int iters, int reps, int wg, int ncu, int mode,
+                         int32_t *out, long n_words) {
+  if (!out || nw <= 0 || ncu <= 0 || wg <= 0 || L < 0 || iters < 0 || reps <= 0 ||
+      (mode != 0 && mode != 1) || !hs_prim_supported(p))
+    return HS_EINVAL;
+  const long n = hs_storage_words(p, nw, ncu, wg, L);
+  if (n < 0 || n != n_words) return HS_EINVAL;
+  const long ir = (long)iters * reps;
+  const int nab = nw < ncu ? nw : ncu;
+  const int per = (nw + nab - 1) / nab;
+  long writers = 0;  // semaphores: one writer per group (main.hip.cpp:883)
+  for (int b = 0; b < nw; ++b) writers += (b / nab == (b % nab) % per);
+  std::vector<long> k;
+  if (p < 4 || p >= 37) {  // barriers: every region, twice with local exchange
+    const bool lx = p == 1 || p == 3 || p == 38 || p == 40 || p == 42;
+    k.assign(nw, ir * ((lx && per > 1) ? 2 : 1));
+  } else if (p <= 7) {
+    k.assign(1, (long)nw * ir);
+  } else if (p <= 19) {  // global semaphores: regions 0..perCU-1
+    k.assign(per, writers * ir);
+  } else if (p >= 32) {  // the global EBO priority kernel over num_wgs regions
+    k.assign(nw, 0);
+    for (int r = 0; r < per && r < nw; ++r) k[r] = writers * ir;
+  } else if (p <= 23) {  // uniq mutexes: one region per group
+    k.assign(nab, 0);
+    for (int b = 0; b < nw; ++b) k[b % ncu] += ir;
+  } else {  // uniq semaphores: each writer walks its group's regions
+    k.assign(nw, 0);
+    for (int b = 0; b < nw; ++b) {
+      if (b / nab != (b % nab) % per) continue;
+      int cur = b;
+      for (int j = 0; j < per; ++j) {
+        k[cur] += ir;
+        cur = (cur + nab) % nw;
+      }
+    }
+  }
+  const long region = (long)wg * (L + 1);
+  if ((long)k.size() * region != n) return HS_EINVAL;
+  const long chain = 16L * (L + 1);
+  for (long i = 0; i < n; ++i) {
+    const long kk = k[i / region];
+    const long c0 = (i / chain) * chain, j = (i - c0) / 16, w = (i - c0) % 16;
+    out[i] = (int32_t)(mode == 0 ? c0 + (j - kk > 0 ? j - kk : 0) * 16 + w
+                                 : i + (j >= 1 ? kk : 0));
+  }
+  return HS_OK;
+}
+
 }  // extern "C"

@@ -575,10 +575,10 @@ extern "C" int hs_refcounter_run(hs_ctx *c, int sharers, int num_tbs, int tb, in
   return HS_OK;
 }
 
-extern "C" int hs_splitcounter_run(hs_ctx *c, int num_tbs, int tb, int iters,
+extern "C" int hs_splitcounter_run(hs_ctx *c, int num_tbs, int tb, int iters, int read_every,
                                    uint32_t *counters, uint32_t *out, hs_relaxed_result *res) {
   if (!c || !counters || !out || !res || num_tbs < 1 || tb < 128 || tb % 128 || tb > 1024 ||
-      iters < 0)
+      iters < 0 || read_every < 0)
     return HS_EINVAL;
   std::memset(res, 0, sizeof *res);
   HS_HIP(hipSetDevice(c->device));
@@ -595,7 +595,10 @@ extern "C" int hs_splitcounter_run(hs_ctx *c, int num_tbs, int tb, int iters,
   HS_HIP(hipMemset(d_part, 0, num_tbs * 4));
   HS_HIP(hipMemset(d_out, 0, nthr * 4));
   HS_HIP(hipMemset(d_stats, 0, 32));
-  const int read_every = iters >= 64 ? 64 : (iters > 0 ? iters : 1);
+  // every WG reads at least once: the period is capped at the run's length
+  const int longest = iters > 0 ? iters : 1;
+  if (read_every == 0) read_every = longest < 64 ? longest : 64;
+  if (read_every > longest) read_every = longest;
   SplitArgs a{d_cnt, d_part, d_out, d_stats, iters, read_every};
   if ((rc = load_kernel((const void *)k_splitcounter))) return rc;  // not inside the timed launch
   Timer t;

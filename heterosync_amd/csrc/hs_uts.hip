@@ -54,6 +54,8 @@ constexpr int kWave = 64;
 constexpr int kMaxSlots = 64;
 constexpr int kMaxThr = 128;
 constexpr long kGrow = 1024;  // deque size below which a WG pops oldest first
+constexpr int kDefaultDeque = 1 << 16;  // records per WG deque by default
+constexpr int kMinDeque = 64, kMaxDeque = 1 << 24;
 
 // ---------------------------------------------------------------------------
 // SHA-1 (FIPS 180-4) of one padded 64-byte block.  The round functions and
@@ -141,9 +143,18 @@ struct Rec<1> {
 };
 
 struct UtsArgs {
-  // deques: num_blocks rings of dq_cap records
+  // deques: num_blocks rings of 2^dq_log2 records
   uint32_t *deq;
   int dq_log2;
+  // Bounded deques (set per run).  A WG never holds more than dq_cap
+  // records.  A one-record-at-a-time depth-first walk from any record adds
+  // at most depth_limit * (max_children - 1) records, so every WG keeps that
+  // reserve free: a batch, a delivery or a breadth-first pop that would
+  // leave more than dq_soft = dq_cap - reserve records is cut to what fits,
+  // down to a single depth-first pop, which the reserve always covers.  No
+  // deque overflows, so nothing aborts.  box_lim (<= dq_soft) caps a
+  // donation, so a delivery to an empty deque always fits.
+  int dq_cap, dq_soft, box_lim;
   // work requests: nrq rings; ring r has its head and tail in one 64-bit
   // word at rq[r * 32] (head low) and rq_mask + 1 entries at
   // rq_ring[r * (rq_mask + 1)] holding requester ids + 1 (0 = not yet
@@ -170,7 +181,9 @@ struct UtsArgs {
   const uint32_t *value;
   const int32_t *child_a, *child_b;
   // control
-  uint32_t *ctl;     // [0] abort, [32] deque overflow
+  uint32_t *ctl;     // [0] abort, [32] deque overflow (a bug: never set),
+                     // [40] batches cut by the deque bound, [48] the most
+                     // records any WG held
   uint64_t ticks;
   // serving (null when not serving)
   uint32_t *live;          // number of live (admitted, unfinished) trees
@@ -191,6 +204,14 @@ struct UtsArgs {
 };
 
 constexpr int kStatWords = 12;
+
+// Most children one record can have: the geometric count is capped by the
+// threshold table (at most 100, the UTS MAXNUMCHILDREN); the reference tree
+// is binary.
+template <int TREE>
+__device__ __forceinline__ int max_children(const UtsArgs &a) {
+  return TREE == 1 ? a.nthr : 2;
+}
 
 __device__ __forceinline__ uint32_t meta_slot(uint32_t m) { return m >> 16; }
 __device__ __forceinline__ uint32_t meta_depth(uint32_t m) { return m & 0xffffu; }
@@ -303,8 +324,9 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   __shared__ uint32_t s_thr[kMaxThr];
   __shared__ uint32_t s_gmx[kMaxSlots];   // depth limit of each slot's tree
   const int lane = threadIdx.x;
-  const long dq_cap = 1L << a.dq_log2, dq_mask = dq_cap - 1;
-  uint32_t *deq = a.deq + (long)blockIdx.x * (dq_cap * W + kDqSkew);
+  const long dq_ring = 1L << a.dq_log2, dq_mask = dq_ring - 1;
+  const long dq_cap = a.dq_cap, dq_soft = a.dq_soft;
+  uint32_t *deq = a.deq + (long)blockIdx.x * (dq_ring * W + kDqSkew);
   Guard g;
   g.init(a.ctl, a.ticks);
   if (SERVE && blockIdx.x == gridDim.x - 1) {  // the dispatcher, not a worker
@@ -329,6 +351,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   uint32_t idle_iter = 0, donate_checks = 0;
   const int me = blockIdx.x;
   uint32_t *my_n = a.wbox_n + (long)me * HS_LINE_WORDS;
+  uint32_t cut_batches = 0, high_water = 0;  // deque bound: batches cut, most held
   unsigned long long st_busy = 0, st_idle = 0, st_nodes = 0, st_don = 0, st_donr = 0,
                      st_steal = 0, st_stolen = 0, st_tpop = 0, st_tgen = 0, st_tdon = 0,
                      st_rounds = 0;
@@ -349,6 +372,10 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
     t_mark = now;
   };
   auto flush_stats = [&]() {
+    if (lane == 0) {
+      if (cut_batches) add_rlx(a.ctl + 40, cut_batches);
+      if (high_water) __hip_atomic_fetch_max(a.ctl + 48, high_water, __ATOMIC_RELAXED, HS_AGENT);
+    }
     if (!STATS || lane != 0) return;
     unsigned long long *o = a.stats + (long)blockIdx.x * kStatWords;
     o[0] = st_busy;
@@ -416,15 +443,12 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   };
 
   // Takes the k records a donor (or the host, for a root) left in this WG's
-  // inbox; their credits come with them.  Whole wave.
+  // inbox; their credits come with them.  A delivery that would take the
+  // deque past dq_soft waits in the inbox (false) until the WG has worked
+  // its deque down; k <= box_lim <= dq_soft, so an empty deque always
+  // takes it.  Whole wave.
   auto absorb = [&](int k) -> bool {
-    if (top - bot + k > dq_cap) {  // no room: report a deque overflow
-      if (lane == 0) {
-        st_rlx(a.ctl + 32, 1);
-        st_rlx(a.ctl, 1);
-      }
-      return false;
-    }
+    if (top - bot + k > dq_soft) return false;
     flush_top();  // the delivery lands above the current top
     fence_acq();  // the donor released the records with the count
     if (SERVE)
@@ -477,7 +501,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
     }
     v = __builtin_amdgcn_readlane(v, 0);
     if (v < 0) return;
-    const int k = (int)std::min<long>((top - bot) / 2, (long)a.wbox_cap);
+    const int k = (int)std::min<long>((top - bot) / 2, (long)a.box_lim);
     uint32_t *box = a.wbox + (long)v * a.wbox_cap * W;
     for (int i0 = 0; i0 < k; i0 += kWave) {
       const int i = i0 + lane;
@@ -523,8 +547,10 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
       // donatable frontier; oldest first (breadth-first) while it is
       // smaller, so that it grows into one: the frontier is what donations
       // are cut from.
-      const int n = held < kWave ? (int)held : kWave;
-      const bool bfs = held < kGrow;
+      int n = held < kWave ? (int)held : kWave;
+      // breadth-first only while a whole batch of the widest fan-out stays
+      // under the deque bound, so a breadth-first batch is never cut
+      const bool bfs = held < kGrow && held + (long)kWave * max_children<TREE>(a) <= dq_soft;
       if (bfs) flush_top();  // pops from the bottom; this batch refills s_top
       uint32_t nc = 0, meta = 0, slot = 0, dep = 0, lval = 0;
       if (lane < n) {
@@ -564,7 +590,27 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
       }
       // child offsets (exclusive scan) and the batch total
       uint32_t total;
-      const uint32_t off = wave_excl_scan(lane < n ? nc : 0, &total);
+      uint32_t off = wave_excl_scan(lane < n ? nc : 0, &total);
+      if (!bfs && held - n + (long)total > dq_soft) {
+        // Deque bound: pop only the newest parents whose children keep the
+        // deque at or under dq_soft (a prefix of the lanes), at least one.
+        // A single depth-first pop always fits: every record's subtree,
+        // walked one record at a time, fits in the reserve above dq_soft.
+        const bool over = lane < n && held - (lane + 1) + (long)(off + nc) > dq_soft;
+        const unsigned long long bad = __ballot(over);
+        int fit = bad ? (int)__builtin_ctzll(bad) : n;
+        if (fit == 0) fit = 1;
+        // s_top records this batch no longer pops go back to their deque
+        // slots [top - lds_n, top - fit)
+        if (lane < lds_n - fit) {
+          const long at = (top - lds_n + lane) & dq_mask;
+#pragma unroll
+          for (int w = 0; w < W; ++w) deq[dq_idx<W>(at, w)] = s_top[w * kWave + lane];
+        }
+        n = fit;
+        off = wave_excl_scan(lane < n ? nc : 0, &total);
+        ++cut_batches;
+      }
       s_off[lane] = off;
       if (lane == 63) s_off[64] = off + (lane < n ? nc : 0);
       // per-tree accounting: one lane when the batch is all one tree (the
@@ -598,9 +644,9 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
         }
       }
       if (bfs) bot += n; else top -= n;
-      lds_n = 0;  // a depth-first pop of n >= lds_n took them all
+      lds_n = 0;  // popped, or written back to the deque above
       __syncthreads();
-      if (top + total - bot > dq_cap) {  // would overflow the ring
+      if (top + total - bot > dq_cap) {  // the bound failed: never taken
         if (lane == 0) {
           st_rlx(a.ctl + 32, 1);
           st_rlx(a.ctl, 1);
@@ -663,6 +709,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
       }
       top += total;
       lds_n = (int)keep;
+      if (top - bot > (long)high_water) high_water = (uint32_t)(top - bot);
       ++batches;
       st_nodes += n;
       __syncthreads();
@@ -670,7 +717,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
       // ---- hand-offs: a late delivery in, the oldest half out ---------------
       {
         const int k_in = (int)__builtin_amdgcn_readlane(box_n, 0);
-        if (k_in > 0 && !absorb(k_in)) return;
+        if (k_in > 0) absorb(k_in);  // waits in the inbox while it does not fit
       }
       if ((batches & 1) == 0 && top - bot >= 2 * kWave) donate();
       // return the credits of trees this WG no longer holds, so that a tree
@@ -689,10 +736,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
     settle();
     {
       const int k_in = (int)__builtin_amdgcn_readlane(lane == 0 ? ld_rlx(my_n) : 0u, 0);
-      if (k_in > 0) {
-        if (!absorb(k_in)) return;
-        continue;
-      }
+      if (k_in > 0 && absorb(k_in)) continue;  // an empty deque takes any delivery
     }
     // admit the next submitted request, if any.  The request counter lives
     // in host memory: each idle WG looks every 4th pass, staggered
@@ -797,12 +841,6 @@ k_uts_stats(UtsArgs a) {
 // nothing from a bigger grid), so it keeps its full register allocation.
 __global__ void __launch_bounds__(64) k_uts_serve(UtsArgs a) { uts_worker<1, true, false>(a); }
 
-__global__ void k_root_digest(uint32_t seed, uint32_t *out) {
-  uint32_t st[5];
-  uts_root(seed, st);
-  for (int i = 0; i < 5; ++i) out[i] = st[i];
-}
-
 }  // namespace
 
 // ===========================================================================
@@ -840,7 +878,7 @@ int geo_thresholds(double b0, std::vector<uint32_t> &thr) {
 struct Engine {
   DevArena ar;
   UtsArgs a;
-  int tree = -1, blocks = 0, W = 0, dq_log2 = 0;
+  int tree = -1, blocks = 0, W = 0, dq_log2 = 0, cap = 0;
   double b0 = 0;
   uint32_t *d_thr = nullptr;
   uint32_t *state = nullptr;
@@ -848,21 +886,26 @@ struct Engine {
   uint32_t *results = nullptr;  // kResultWords words inside `state`
   static constexpr size_t kResultWords = 3 * 2 * kMaxSlots + kMaxSlots + 64;
 
-  bool matches(int t, int nb, int dq, double b) const {
-    return tree == t && blocks == nb && dq_log2 == dq && (t == 0 || b0 == b);
+  bool matches(int t, int nb, int cap_, double b) const {
+    return tree == t && blocks == nb && cap == cap_ && (t == 0 || b0 == b);
   }
 
-  // tree: 0 reference, 1 geometric; serve: the persistent serving kernel
-  int setup(hs_ctx *c, int tree_, int num_blocks, int dq_log2_, double b0_) {
+  // tree: 0 reference, 1 geometric.  cap_: records per WG deque (the ring
+  // is the next power of two).
+  int setup(hs_ctx *c, int tree_, int num_blocks, int cap_, double b0_) {
     std::memset(&a, 0, sizeof a);
+    if (cap_ < kMinDeque || cap_ > kMaxDeque) return HS_EINVAL;
     tree = tree_;
-    dq_log2 = dq_log2_;
+    cap = cap_;
+    dq_log2 = 0;
+    while ((1L << dq_log2) < cap) ++dq_log2;
     b0 = b0_;
     blocks = num_blocks > 0 ? num_blocks : 8 * c->num_cu;
     W = tree == 1 ? 6 : 2;
     a.dq_log2 = dq_log2;
-    // one inbox holds at most a quarter deque, so a late delivery fits
-    a.wbox_cap = (int)std::min<long>(4096, (1L << dq_log2) / 4);
+    a.dq_cap = cap;
+    // an inbox holds at most a quarter deque
+    a.wbox_cap = (int)std::min<long>(4096, cap / 4);
     // request rings: up to 32; every WG has at most one request outstanding
     // on its ring, and a claimed entry may be cleared late: twice the ring's
     // WG count keeps a new request off it
@@ -913,6 +956,20 @@ struct Engine {
     return HS_OK;
   }
 
+  // The deque bound of a run whose trees go at most `depth` levels below
+  // their roots: a one-record-at-a-time depth-first walk from any record
+  // adds at most depth * (max_children - 1) records, and every WG keeps
+  // that much of its deque free.  HS_EINVAL when the deque cannot hold even
+  // that reserve.
+  int bound(long depth) {
+    const long kids = tree == 1 ? a.nthr : 2;
+    const long soft = (long)cap - depth * (kids - 1);
+    if (soft < 1) return HS_EINVAL;
+    a.dq_soft = (int)soft;
+    a.box_lim = (int)std::min<long>(a.wbox_cap, soft);
+    return HS_OK;
+  }
+
   // Clears the per-run state on the stream (never inside a captured graph).
   int reset(hipStream_t s) {
     HS_HIP(hipMemsetAsync(state, 0, state_words * 4, s));
@@ -933,7 +990,7 @@ struct Engine {
     r->leaf_sum = u64(2 * kMaxSlots + slot);
     r->max_depth = (int)res[6 * kMaxSlots + slot];
     r->aborted = ctl[0] != 0;
-    if (ctl[32]) return HS_ENOMEM;  // a deque ring overflowed
+    if (ctl[32]) return HS_ENOMEM;  // a deque ring overflowed (the bound failed)
     return ctl[0] ? HS_ETIMEOUT : HS_OK;
   }
 
@@ -1016,9 +1073,12 @@ int stats_report(Engine &e, double kernel_ms) {
 }
 
 // One run: clear the state, deliver the root, run the kernel, read slot 0.
+// depth: how many levels the tree goes below its root (the deque bound).
 template <int TREE>
-int run_single(hs_ctx *c, Engine &e, uint32_t seed, uint32_t gen_mx, hs_uts_result *res) {
+int run_single(hs_ctx *c, Engine &e, uint32_t seed, uint32_t gen_mx, long depth,
+               hs_uts_result *res) {
   int rc;
+  if ((rc = e.bound(depth))) return rc;
   if ((rc = e.reset(c->stream))) return rc;
   if ((rc = stats_begin(e, c->stream))) return rc;
   hipLaunchKernelGGL(k_seed_root, dim3(1), dim3(kWave), 0, c->stream, e.a, TREE, seed, gen_mx);
@@ -1054,9 +1114,26 @@ extern "C" int hs_uts_ref_run(hs_ctx *c, int num_nodes, const uint32_t *value,
     if (child_a[i] < 0 || child_a[i] >= num_nodes || child_b[i] < 0 || child_b[i] >= num_nodes ||
         (child_a[i] == 0) != (child_b[i] == 0))
       return HS_EINVAL;
+  // the tree's height bounds the deques (binary: one extra record per level)
+  long height = 0;
+  {
+    std::vector<int32_t> depth(num_nodes, -1), queue(1, 0);
+    depth[0] = 0;
+    for (size_t q = 0; q < queue.size(); ++q) {
+      const int v = queue[q];
+      if (child_a[v] == 0) continue;
+      for (const int ch : {child_a[v], child_b[v]}) {
+        if (depth[ch] >= 0) return HS_EINVAL;  // not a tree
+        depth[ch] = depth[v] + 1;
+        height = std::max<long>(height, depth[ch]);
+        queue.push_back(ch);
+      }
+    }
+  }
   HS_HIP(hipSetDevice(c->device));
   Engine e;
-  int rc = e.setup(c, 0, num_blocks, 14, 0);
+  const int cap = (int)std::min<long>(kMaxDeque, std::max<long>(1L << 14, height + 4096));
+  int rc = e.setup(c, 0, num_blocks, cap, 0);
   if (rc) return rc;
   uint32_t *d_val;
   int32_t *d_a, *d_b;
@@ -1069,7 +1146,7 @@ extern "C" int hs_uts_ref_run(hs_ctx *c, int num_nodes, const uint32_t *value,
   e.a.value = d_val;
   e.a.child_a = d_a;
   e.a.child_b = d_b;
-  return run_single<0>(c, e, 0, 0, res);
+  return run_single<0>(c, e, 0, 0, height, res);
 }
 
 // The geometric engine of a context is kept between calls (same grid, deque
@@ -1083,12 +1160,12 @@ extern "C" int hs_uts_geo_run(hs_ctx *c, int seed, double b0, int gen_mx, int nu
   HS_HIP(hipSetDevice(c->device));
   const int nb = num_blocks > 0 ? num_blocks : Engine::default_blocks(c, b0, gen_mx);
   Engine *e = static_cast<Engine *>(c->uts_cache);
-  if (!e || !e->matches(1, nb, 16, b0)) {
+  if (!e || !e->matches(1, nb, kDefaultDeque, b0)) {
     if (c->uts_cache_free) c->uts_cache_free(c->uts_cache);
     c->uts_cache = nullptr;
     c->uts_cache_free = nullptr;
     e = new Engine;
-    int rc = e->setup(c, 1, nb, 16, b0);
+    int rc = e->setup(c, 1, nb, kDefaultDeque, b0);
     if (rc) {
       delete e;
       return rc;
@@ -1096,7 +1173,7 @@ extern "C" int hs_uts_geo_run(hs_ctx *c, int seed, double b0, int gen_mx, int nu
     c->uts_cache = e;
     c->uts_cache_free = free_engine;
   }
-  return run_single<1>(c, *e, (uint32_t)seed, (uint32_t)gen_mx, res);
+  return run_single<1>(c, *e, (uint32_t)seed, (uint32_t)gen_mx, gen_mx, res);
 }
 
 extern "C" int hs_uts_serve(hs_ctx *c, const hs_uts_request *req, int n, const hs_serve_cfg *cfg,
@@ -1109,15 +1186,24 @@ extern "C" int hs_uts_serve(hs_ctx *c, const hs_uts_request *req, int n, const h
   std::memset(sum, 0, sizeof *sum);
   std::memset(resp, 0, sizeof(*resp) * n);
   const int slots = cfg->slots > 0 ? std::min(cfg->slots, kMaxSlots) : 8;
+  if (cfg->num_blocks < 0 || cfg->deque_nodes < 0) return HS_EINVAL;
+  const int blocks = cfg->num_blocks > 0 ? cfg->num_blocks : 8 * c->num_cu;
+  if (blocks < 2) return HS_EINVAL;  // the dispatcher and at least one worker
   HS_HIP(hipSetDevice(c->device));
   int rc;
   {  // persistent: every WG, the dispatcher (the grid's last) included, must be resident
     int fit = 0;
     if ((rc = coresident_wgs(c, (const void *)k_uts_serve, 64, &fit))) return rc;
-    if (cfg->num_blocks > fit) return HS_EOCCUPANCY;
+    if (blocks > fit) return HS_EOCCUPANCY;
   }
   Engine e;
-  if ((rc = e.setup(c, 1, cfg->num_blocks, 16, b0))) return rc;
+  const int cap = cfg->deque_nodes > 0 ? cfg->deque_nodes : kDefaultDeque;
+  if ((rc = e.setup(c, 1, blocks, cap, b0))) return rc;
+  {  // every WG keeps the reserve of the deepest tree of the session
+    long dmax = 0;
+    for (int i = 0; i < n; ++i) dmax = std::max<long>(dmax, req[i].gen_mx);
+    if ((rc = e.bound(dmax))) return rc;
+  }
   if (cfg->timeout_s > 0) e.a.ticks = timeout_ticks(c, cfg->timeout_s);
   // pinned, host-coherent request inbox / response board / stop word
   struct Pinned {
@@ -1140,7 +1226,6 @@ extern "C" int hs_uts_serve(hs_ctx *c, const hs_uts_request *req, int n, const h
   e.a.stop = h_ctl + 32;
   // device copies of the requests and control words (one dispatcher WG,
   // the grid's last, keeps them; the others are workers)
-  if (e.blocks < 2) return HS_EINVAL;
   if ((rc = e.ar.alloc(&e.a.dev_inbox, (size_t)n * 4))) return rc;
   if ((rc = e.ar.alloc(&e.a.dev_ctl, 64))) return rc;
   HS_HIP(hipMemset(e.a.dev_inbox, 0, (size_t)n * 16));
@@ -1207,6 +1292,9 @@ extern "C" int hs_uts_serve(hs_ctx *c, const hs_uts_request *req, int n, const h
   sum->wall_s = now_s();
   uint32_t ctl[64];
   HS_HIP(hipMemcpy(ctl, e.a.ctl, sizeof ctl, hipMemcpyDeviceToHost));
+  sum->cut_batches = ctl[40];
+  sum->deque_high_water = ctl[48];
+  sum->deque_soft = (uint32_t)e.a.dq_soft;
   if (rc == HS_OK && ctl[32]) rc = HS_ENOMEM;
   if (rc == HS_OK && ctl[0]) rc = HS_ETIMEOUT;
   std::vector<double> v(lat.begin(), lat.begin() + done);

@@ -12,6 +12,8 @@
  *                              hip/syncPrims/uvm/main.hip.cpp:1995-2011
  *   hs_prim_id / hs_prim_name  the string->id table main.hip.cpp:2036-2089
  *   hs_storage_words           numLocsMult/numStorageLocs main.hip.cpp:2109-2136
+ *   hs_syncprim_expected       the host golden and its compare,
+ *                              main.hip.cpp:2524-2702
  *   hs_syncprim_run            hipMutexCreate* / hipSemaphoreCreate* +
  *                              the ROI and invoke* launchers
  *                              main.hip.cpp:1423-1927, :2354-2511
@@ -118,6 +120,15 @@ typedef struct {
 int hs_syncprim_run(hs_ctx *ctx, const hs_syncprim_cfg *cfg, int32_t *storage,
                     long n_words, hs_syncprim_result *res);
 
+/* The storage a correct run leaves: the reference binary's own self-check
+ * (the host golden of main.hip.cpp:2524-2702), in closed form.  Fills
+ * out[0..n_words) for a run of hs_syncprim_run with these arguments and
+ * storage initialised to storage[i] = i (main.hip.cpp:2141).  Host code
+ * only; needs no device. */
+int hs_syncprim_expected(int prim, int num_ldst, int num_wgs, int num_cs_iters,
+                         int reps, int wg_size, int num_cu, int mode,
+                         int32_t *out, long n_words);
+
 /* ---- relaxed atomics ---------------------------------------------------- */
 
 typedef struct {
@@ -161,12 +172,16 @@ int hs_refcounter_run(hs_ctx *ctx, int sharers, int num_tbs, int tb_size,
                       uint32_t *c1, uint32_t *del0, uint32_t *del1,
                       hs_relaxed_result *res);
 
-/* splitCounter: per wave, lanes 0-31 are writers (one atomicAdd(+1) per
- * iteration on their own counter), lanes 32-63 are readers (sum of all
- * counters of the reader's CU partition plus the global approximate total).
- * counters: num_tbs*tb_size/2 words; out_arr num_tbs*tb_size words. */
+/* splitCounter: even waves of a WG are writers (one atomicAdd(+1) per
+ * iteration on their own counter); odd waves are readers.  The read is
+ * WG-collective (splitCounter_kernel.cu:33-76): the first reader wave of
+ * each WG makes it every `read_every` writer iterations (0 = min(64,
+ * iters); capped at iters, so every WG reads at least once), summing the WG's counters into its per-WG partial and then every
+ * partial into the approximate global total, which every reader thread
+ * outputs.  iters / read_every reads per WG in all.  counters:
+ * num_tbs*tb_size/2 words; out_arr num_tbs*tb_size words. */
 int hs_splitcounter_run(hs_ctx *ctx, int num_tbs, int tb_size, int iters,
-                        uint32_t *counters, uint32_t *out_arr,
+                        int read_every, uint32_t *counters, uint32_t *out_arr,
                         hs_relaxed_result *res);
 
 /* ---- UTS ----------------------------------------------------------------- */
@@ -182,13 +197,18 @@ typedef struct {
 
 /* Reference binary tree (uts.cu:290-339 layout: value, child_a, child_b per
  * node; child_a == 0 means leaf).  Work-stealing over per-WG deques in
- * HBM; num_blocks WGs of 64 lanes (0 = 2 per CU). */
+ * HBM; num_blocks WGs of 64 lanes (0 = 8 per CU, two waves per SIMD).
+ * Each deque holds max(2^14, tree height + 4096) records. */
 int hs_uts_ref_run(hs_ctx *ctx, int num_nodes, const uint32_t *value,
                    const int32_t *child_a, const int32_t *child_b,
                    int num_blocks, hs_uts_result *res);
 
 /* One geometric (fixed shape) SHA-1 tree: root seed, expected branching
- * b0, depth limit gen_mx (UTS -t 1 -a 3 -b b0 -d gen_mx -r seed). */
+ * b0, depth limit gen_mx (UTS -t 1 -a 3 -b b0 -d gen_mx -r seed).
+ * num_blocks 0 = 8 per CU, or every WG that can be resident (four waves
+ * per SIMD) for trees expected above 2.5e8 nodes.  Deques hold 2^16
+ * records; HS_EINVAL when gen_mx * (max_children - 1) exceeds that (the
+ * deque bound, see hs_uts_serve: gen_mx <= 661 at b0 = 4). */
 int hs_uts_geo_run(hs_ctx *ctx, int seed, double b0, int gen_mx,
                    int num_blocks, hs_uts_result *res);
 
@@ -202,12 +222,22 @@ typedef struct {
 } hs_batch_instance;
 
 typedef struct {
-  int wg_size;       /* 0 = 64 */
+  int wg_size;       /* lanes per persistent worker WG: 64, 128 or 256
+                        (0 = 64).  Every virtual WG keeps the reference's
+                        64 threads (main.hip.cpp:9); a wider worker splits
+                        each critical section's lines over its waves. */
   int reps;          /* 0 = 10 */
-  int mode;          /* 0 copy, 1 inc */
-  int backoff_min, backoff_max, poll_sleep;
+  int mode;          /* 0 = copy chain (reference), 1 = in-place increment
+                        (count-exact: a lost or doubled pass changes the
+                        final storage) */
+  int backoff_min;   /* EBO backoff start I_min, in skipped polls x 4;
+                        0 = 1 (hipLocksMutexEBO.h:26) */
+  int backoff_max;   /* EBO cap I_max (2^k-1); 0 = 1023 */
+  int poll_sleep;    /* s_sleep quanta after a pass with no progress;
+                        <0 = 0, 0 = 1 */
   int use_graph;     /* capture the launches in a hipGraph */
-  int persistent_wgs; /* workers of the persistent grid; 0 = auto */
+  int persistent_wgs; /* worker WGs of the persistent grid; 0 = 8 waves
+                         per CU */
   double timeout_s;
 } hs_batch_cfg;
 
@@ -226,9 +256,12 @@ int hs_batch_run(hs_ctx *ctx, const hs_batch_instance *inst, int n,
                  int64_t *entries, hs_batch_result *res);
 
 /* Autotuner for hs_batch_run: times a calibration copy of the batch (CS
- * iterations capped at 20, one launch) over workers per CU {2,4,8,16} x
- * poll interval {0,1,4} x EBO cap {63,255,1023} and writes the fastest
- * persistent_wgs / poll_sleep / backoff_max into *cfg. */
+ * iterations capped at 20, one launch, in cfg->mode) over waves per CU
+ * {2,4,8,16} x poll interval {0,1,4} x EBO cap I_max {63,255,1023} with
+ * 64-lane workers, then, at the winning waves per CU, worker width
+ * {64,128,256} x backoff start I_min {1,4,16}.  Writes the fastest
+ * wg_size / persistent_wgs / poll_sleep / backoff_min / backoff_max into
+ * *cfg (grids that cannot be co-resident are skipped). */
 int hs_batch_autotune(hs_ctx *ctx, const hs_batch_instance *inst, int n,
                       hs_batch_cfg *cfg, double *best_ms);
 
@@ -246,9 +279,12 @@ typedef struct {
 } hs_uts_response;
 
 typedef struct {
-  int slots;          /* concurrent trees on the device; 0 = 8 */
-  int num_blocks;     /* persistent workers; 0 = 2 per CU */
-  int deque_nodes;    /* per-worker deque capacity (nodes); 0 = auto */
+  int slots;          /* concurrent trees on the device (the response
+                         board); 0 = 8, at most 64 */
+  int num_blocks;     /* persistent WGs including the dispatcher; 0 = 8 per
+                         CU (two waves per SIMD).  HS_EOCCUPANCY if they
+                         cannot all be resident. */
+  int deque_nodes;    /* records per WG deque (HBM); 0 = 65536 */
   double timeout_s;
 } hs_serve_cfg;
 
@@ -256,8 +292,20 @@ typedef struct {
   double wall_s;
   uint64_t total_nodes;
   double p50_s, p99_s;
+  uint32_t cut_batches;      /* batches the deque bound cut short */
+  uint32_t deque_high_water; /* most records any WG's deque held */
+  uint32_t deque_soft;       /* the bound: deque_nodes minus the reserve */
 } hs_serve_summary;
 
+/* UTS serving loop: one persistent launch admits the requests as they
+ * arrive (arrival_s after the call starts) into free slots and reports
+ * each tree when its last record is done.  Admission is by deque memory:
+ * a tree is taken by a worker whose deque is empty, and every worker keeps
+ * free the records a one-node-at-a-time depth-first walk of the deepest
+ * tree of the session could add, max gen_mx * (max_children - 1), cutting
+ * batches and holding back deliveries to stay under deque_nodes minus that
+ * reserve.  No deque overflows and no tree is dropped.  HS_EINVAL if
+ * deque_nodes cannot hold the reserve. */
 int hs_uts_serve(hs_ctx *ctx, const hs_uts_request *req, int n,
                  const hs_serve_cfg *cfg, hs_uts_response *resp,
                  hs_serve_summary *sum);

@@ -9,24 +9,26 @@
 
 Every measurement is parity-checked (oracle golden / closed form / exact
 counts) and reported next to its unit rate.  Usage:
-  python -m heterosync_amd.suite [c1 c2 c3 c4 c5]
+  python suite.py [c1 c2 c3 c4 c5]
 """
 from __future__ import annotations
 
 import ctypes
 import json
+import os
 import sys
 import time
 
 import numpy as np
 
-import heterosync_amd as hs
-from heterosync_amd.mix import make_mix
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import heterosync_amd as hs  # noqa: E402
+from heterosync_amd.mix import make_mix  # noqa: E402
 
 
 def _oracle():
-    from tests.conftest import OracleHelpers, _load_oracle
-    o = _load_oracle()
+    from oracle.loader import OracleHelpers, load_oracle
+    o = load_oracle()
     return o, OracleHelpers(o)
 
 
@@ -68,10 +70,8 @@ def c3(dev, oh, seed=3, reps=1, tune=True):
     kw = {}
     tuned = None
     if tune:
-        wgs, poll, bmax, best = dev.batch_autotune(mix, reps=reps)
-        kw = dict(persistent_wgs=wgs, poll_sleep=poll, backoff=(0, bmax))
-        tuned = {"persistent_wgs": wgs, "poll_sleep": poll, "backoff_max": bmax,
-                 "calibration_ms": best}
+        tuned = dev.batch_autotune(mix, reps=reps)
+        kw = dev.tuned_kwargs(tuned)
     st = [np.arange(hs.storage_words(p, nw, ncu, 64, L), dtype=np.int32) for p, nw, it, L in mix]
     entries, rb = dev.batch(mix, st, reps=reps, use_graph=1, timeout_s=60.0, **kw)
     ok = rb.all_ok == 1

@@ -7,6 +7,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import heterosync_amd as hs
@@ -64,3 +65,34 @@ def test_cli_usage(cli, args):
     p = subprocess.run([path, *args], capture_output=True, text=True, timeout=60)
     assert p.returncode == 255  # exit(-1), as the reference
     assert "./" in p.stderr
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_library_self_check_equals_oracle(ohelp, mode):
+    """hs_syncprim_expected (the library's closed form of the reference's
+    host golden, used by the command line's PASSED! check and by the
+    measurement harnesses) equals the oracle's closed form at every
+    primitive id, and the oracle's literal golden where that applies."""
+    from tests.test_oracle import PRIO_EBO, SUPPORTED
+    for prim in SUPPORTED:
+        for nw, ncu, wg, L, iters, reps in [(4, 4, 64, 2, 3, 2), (12, 4, 64, 0, 2, 1),
+                                            (3, 4, 128, 1, 5, 2), (64, 16, 64, 3, 2, 3),
+                                            (1024, 256, 256, 10, 1000, 10)]:
+            if hs.storage_words(prim, nw, ncu, wg, L) < 0:
+                continue
+            got = hs.expected_storage(prim, L, nw, iters, reps, wg, ncu, mode)
+            assert np.array_equal(got, ohelp.golden(prim, L, nw, iters, reps, wg, ncu, mode)), \
+                (prim, nw, ncu, wg, L, mode)
+            if nw <= 64 and prim not in PRIO_EBO:
+                lit = ohelp.golden(prim, L, nw, iters, reps, wg, ncu, mode, literal=True)
+                assert np.array_equal(got, lit), (prim, nw, L, mode)
+
+
+def test_library_self_check_rejects_bad_arguments():
+    lib = hs.lib()
+    out = np.zeros(704, np.int32)
+    p = out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    assert lib.hs_syncprim_expected(7, 10, 32, 100, 10, 64, 256, 0, p, 704) == 0
+    assert lib.hs_syncprim_expected(7, 10, 32, 100, 10, 64, 256, 0, p, 703) == -1  # size
+    assert lib.hs_syncprim_expected(36, 10, 32, 100, 10, 64, 256, 0, p, 704) == -1  # reserved
+    assert lib.hs_syncprim_expected(7, 10, 32, 100, 10, 64, 256, 2, p, 704) == -1  # mode

@@ -1,6 +1,13 @@
 """GPU parity of the batched persistent form: every instance of a seeded
 mix ends bit-exact with the oracle (and with its own plain run), with exact
-acquire counts, whether launched plainly or as a captured hipGraph."""
+acquire counts, whether launched plainly or as a captured hipGraph.
+
+Inc mode (in-place increment of every line) makes the storage count-exact:
+with the reference's copy chain every pass past the L-th leaves the same
+array, so only inc mode can see a lost, doubled or overlapping critical
+section.  The full seeded mix of the batched configuration (64 instances,
+numCSIters log-uniform in 10-1000) runs in inc mode, in the default and in
+the autotuned configuration."""
 import numpy as np
 import pytest
 
@@ -15,18 +22,64 @@ def storages_for(dev, mix):
             for p, nw, it, L in mix]
 
 
-@pytest.mark.parametrize("use_graph", [0, 1])
-def test_batch_mix_parity(dev, ohelp, use_graph):
-    mix = make_mix(seed=3, n=64, iters_cap=20)
-    st = storages_for(dev, mix)
-    entries, r = dev.batch(mix, st, reps=3, use_graph=use_graph)
-    assert r.all_ok == 1
+def check_mix(dev, ohelp, mix, st, entries, reps, mode):
     ncu = dev.info.num_cu
     for (p, nw, it, L), s, e in zip(mix, st, entries):
-        assert np.array_equal(s, ohelp.golden(p, L, nw, it, 3, 64, ncu, 0)), (p, nw, it, L)
-        w, rd, _ = ohelp.counts(p, nw, it, 3, ncu)
-        assert e == w + rd
+        assert np.array_equal(s, ohelp.golden(p, L, nw, it, reps, 64, ncu, mode)), \
+            (p, nw, it, L, mode)
+        w, rd, _ = ohelp.counts(p, nw, it, reps, ncu)
+        assert e == w + rd, (p, nw, it, L)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_batch_mix_parity(dev, ohelp, use_graph, mode):
+    mix = make_mix(seed=3, n=64, iters_cap=20)
+    st = storages_for(dev, mix)
+    entries, r = dev.batch(mix, st, reps=3, mode=mode, use_graph=use_graph)
+    assert r.all_ok == 1
+    check_mix(dev, ohelp, mix, st, entries, 3, mode)
     assert r.acquires == int(entries.sum())
+
+
+def test_batch_full_mix_inc(dev, ohelp):
+    """The batched configuration at its stated size: the seed-3 mix of 64
+    instances, numCSIters uncapped (10-1000), one persistent launch, inc
+    mode; every instance's storage count-exact against the oracle."""
+    mix = make_mix(seed=3, n=64)
+    assert max(it for _, _, it, _ in mix) > 500
+    st = storages_for(dev, mix)
+    entries, r = dev.batch(mix, st, reps=1, mode=1, use_graph=1, timeout_s=60.0)
+    assert r.all_ok == 1
+    check_mix(dev, ohelp, mix, st, entries, 1, 1)
+
+
+def test_batch_full_mix_inc_autotuned(dev, ohelp):
+    """The same mix in the configuration the autotuner picks (tuned in inc
+    mode), run twice in a row: the tuned knobs change scheduling only."""
+    mix = make_mix(seed=3, n=64)
+    t = dev.batch_autotune(mix, reps=1, mode=1)
+    assert t["wg_size"] in (64, 128, 256) and t["backoff_min"] in (1, 4, 16)
+    assert t["persistent_wgs"] > 0 and t["best_ms"] > 0
+    for _ in range(2):
+        st = storages_for(dev, mix)
+        entries, r = dev.batch(mix, st, reps=1, mode=1, use_graph=1, timeout_s=60.0,
+                               **dev.tuned_kwargs(t))
+        assert r.all_ok == 1
+        check_mix(dev, ohelp, mix, st, entries, 1, 1)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("wg_size", [128, 256])
+def test_batch_worker_width(dev, ohelp, wg_size, mode):
+    """Workers of 2 and 4 waves split each critical section's lines over
+    their waves (numLdSt 1, 10 and 100 all occur in the mix)."""
+    mix = make_mix(seed=23, n=48, iters_cap=30)
+    assert {L for _, _, _, L in mix} == {1, 10, 100}
+    st = storages_for(dev, mix)
+    entries, r = dev.batch(mix, st, reps=2, mode=mode, wg_size=wg_size, use_graph=1)
+    assert r.all_ok == 1
+    check_mix(dev, ohelp, mix, st, entries, 2, mode)
 
 
 @pytest.mark.parametrize("seed", [17, 29, 41])
@@ -36,29 +89,24 @@ def test_batch_mix_seeds_and_workers(dev, ohelp, seed, workers):
     worker) or many (a few each)."""
     mix = make_mix(seed=seed, n=32, iters_cap=15)
     st = storages_for(dev, mix)
-    entries, r = dev.batch(mix, st, reps=2, use_graph=1, persistent_wgs=workers)
+    entries, r = dev.batch(mix, st, reps=2, mode=1, use_graph=1, persistent_wgs=workers)
     assert r.all_ok == 1
-    ncu = dev.info.num_cu
-    for (p, nw, it, L), s, e in zip(mix, st, entries):
-        assert np.array_equal(s, ohelp.golden(p, L, nw, it, 2, 64, ncu, 0)), (p, nw, it, L)
-        w, rd, _ = ohelp.counts(p, nw, it, 2, ncu)
-        assert e == w + rd
+    check_mix(dev, ohelp, mix, st, entries, 2, 1)
 
 
 def test_batch_alone_equals_plain(dev):
     mix = make_mix(seed=11, n=8, iters_cap=10)
-    st = storages_for(dev, mix)
-    dev.batch(mix, st, reps=2)
-    for (p, nw, it, L), s in zip(mix, st):
-        plain, r = dev.syncprim(p, L, nw, it, reps=2)
-        assert np.array_equal(plain, s)
+    for mode in (0, 1):
+        st = storages_for(dev, mix)
+        dev.batch(mix, st, reps=2, mode=mode)
+        for (p, nw, it, L), s in zip(mix, st):
+            plain, r = dev.syncprim(p, L, nw, it, reps=2, mode=mode)
+            assert np.array_equal(plain, s), (p, nw, it, L, mode)
 
 
-def test_batch_autotune(dev):
-    mix = make_mix(seed=5, n=16, iters_cap=50)
-    wgs, poll, bmax, best = dev.batch_autotune(mix)
-    assert wgs > 0 and best > 0
-    st = storages_for(dev, mix)
-    _, r = dev.batch(mix, st, reps=2, persistent_wgs=wgs, poll_sleep=poll,
-                     backoff=(0, bmax))
-    assert r.all_ok == 1
+def test_batch_rejects_bad_width_and_mode(dev):
+    mix = make_mix(seed=11, n=2, iters_cap=2)
+    for kw in (dict(wg_size=96), dict(wg_size=512), dict(mode=2)):
+        with pytest.raises(hs.HSError) as ei:
+            dev.batch(mix, storages_for(dev, mix), reps=1, **kw)
+        assert ei.value.code == -1  # HS_EINVAL

@@ -417,3 +417,19 @@ def test_cpu_barrier_pool_matches_closed_form(oracle, ohelp, prim):
         t = oracle.hso_cpu_barrier_run(prim, 3, nw, 4, 2, 64, ncu, mode, threads,
                                        got.ctypes.data_as(P32), got.size)
         assert t >= 0 and np.array_equal(got, want), (prim, nw, ncu, threads, mode)
+
+
+def test_cpu_emulation_under_thread_sanitizer():
+    """The std::thread + std::atomic emulation of every primitive, and the
+    threaded barrier pool, built with -fsanitize=thread: no data race is
+    reported and every run lands on the closed form."""
+    import os
+    import subprocess
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    subprocess.run(["make", "-s", "-C", here, "tsan"], check=True, timeout=300)
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 exitcode=66")
+    p = subprocess.run([os.path.join(here, "tsan_emul")], capture_output=True, text=True,
+                       timeout=300, env=env)
+    assert "ThreadSanitizer" not in p.stderr, p.stderr[-2000:]
+    assert p.returncode == 0, p.stdout + p.stderr[-2000:]
+    assert "0 mismatches" in p.stdout

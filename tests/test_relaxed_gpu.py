@@ -24,6 +24,7 @@ def seq_expected(oracle, nw, gs, period, iters, n):
     (512, 128, 4, 16, 10),     # the reference's run file: ./seqlocks 512 128 4 {0,1}
     (256, 64, 16, 65, 200),
     (1024, 256, 16, 65, 50),   # full-occupancy shape, 1 writer WG per 64 readers
+    (1024, 256, 16, 65, 10000),  # relaxed-atomics configuration: 10^4 iterations
     (2048, 64, 8, 33, 100),    # 8 WGs per CU, more writers
     (128, 1024, 2, 16, 20),    # 1024-thread WGs
 ])
@@ -70,6 +71,7 @@ def test_flags_never_dirty(dev):
 @pytest.mark.parametrize("sh,ntb,tb,reps,spacing", [
     (32, 64, 32, 4, -1),            # the reference's run file
     (32, 1024, 256, 100, 8),
+    (32, 1024, 256, 10000, 8),   # relaxed-atomics configuration: 10^4 inc/dec pairs
     (4, 512, 64, 50, 0),
 ])
 def test_refcounter(dev, sh, ntb, tb, reps, spacing, strict):
@@ -82,10 +84,15 @@ def test_refcounter(dev, sh, ntb, tb, reps, spacing, strict):
     assert r.atomic_ops == 4 * ntb * tb * reps
 
 
-@pytest.mark.parametrize("ntb,tb,iters", [(112, 128, 1), (256, 256, 1000), (1024, 256, 200),
-                                         (64, 1024, 300)])
-def test_splitcounter(dev, ntb, tb, iters):
-    counters, out, r = dev.splitcounter(ntb, tb, iters)
+@pytest.mark.parametrize("ntb,tb,iters,read_every", [
+    (112, 128, 1, 0), (256, 256, 1000, 0), (1024, 256, 200, 0), (64, 1024, 300, 0),
+    (1024, 256, 10000, 0),    # relaxed-atomics configuration: 10^4 iterations
+    (1024, 256, 10000, 1),    # a WG read after every writer iteration
+    (1024, 256, 10000, 2500),  # four reads per WG
+    (256, 256, 100, 1000),    # read period past the run: one read at the end
+])
+def test_splitcounter(dev, ntb, tb, iters, read_every):
+    counters, out, r = dev.splitcounter(ntb, tb, iters, read_every)
     assert np.all(counters == iters)
     total = iters * counters.size
     assert out.max() <= total
@@ -95,3 +102,9 @@ def test_splitcounter(dev, ntb, tb, iters):
     # outputs the same sum
     readers = w[:, 1::2, :].reshape(ntb, -1)
     assert np.all(readers == readers[:, :1])
+    # operation count: every writer add, plus each WG's reads (its own
+    # counters' lines, then every WG's partial, plus the partial's store)
+    period = min(read_every if read_every > 0 else 64, max(iters, 1))
+    reads = iters // period
+    per_read = (tb // 2 + 63) // 64 + (ntb + 63) // 64 + 1
+    assert r.atomic_ops == counters.size * iters + ntb * reads * per_read * 64

@@ -79,10 +79,74 @@ def test_uts_serve_alone_equals_mixed(dev, oracle):
 
 def test_uts_serve_rejects_grid_that_cannot_be_resident(dev):
     """The serving loop is persistent (its dispatcher is the grid's last WG),
-    so a grid larger than what can be co-resident is refused before launch."""
+    so a grid larger than what can be co-resident is refused before launch.
+    The default grid (8 WGs per CU) is checked the same way: it runs, and so
+    does the same grid given explicitly."""
     import heterosync_amd as hs
     with pytest.raises(hs.HSError) as ei:
         dev.uts_serve([(1, 3, 4.0, 0.0)], slots=2, num_blocks=64 * dev.info.num_cu)
     assert ei.value.code == -4  # HS_EOCCUPANCY
-    resp, _ = dev.uts_serve([(1, 3, 4.0, 0.0)], slots=2)  # the default grid fits
-    assert resp[0].nodes > 0
+    for nb in (0, 8 * dev.info.num_cu):
+        resp, _ = dev.uts_serve([(1, 3, 4.0, 0.0)], slots=2, num_blocks=nb)
+        assert resp[0].nodes > 0
+
+
+def poisson_trees(seed, n, dlo, dhi, mean_gap_s):
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for _ in range(n):
+        out.append((int(rng.integers(0, 1 << 30)), int(rng.integers(dlo, dhi + 1)), 4.0, t))
+        t += rng.exponential(mean_gap_s)
+    return out
+
+
+def test_uts_serve_stated_depths(dev, oracle):
+    """The serving configuration's tree shapes: geometric b0 = 4, depth
+    limit uniform in 6-14, Poisson arrivals.  Every tree's counts equal its
+    own alone run, and the oracle's exhaustive search where the tree is
+    small enough for the CPU (depth <= 11, at most 5e6 nodes)."""
+    reqs = poisson_trees(77, 48, 6, 14, 0.002)
+    assert {d for _, d, _, _ in reqs} >= {6, 14}
+    resp, summ = dev.uts_serve(reqs, slots=16, timeout_s=120.0)
+    checked = 0
+    for (seed, d, b0, _), r in zip(reqs, resp):
+        alone = dev.uts_geo(seed, b0, d)
+        assert (alone.nodes, alone.leaves, alone.max_depth) == (r.nodes, r.leaves, r.max_depth)
+        if d <= 11 and alone.nodes <= 5_000_000:
+            assert (r.nodes, r.leaves, r.max_depth) == geo_count(oracle, seed, b0, d)
+            checked += 1
+    assert checked >= 10
+    assert summ.total_nodes == sum(r.nodes for r in resp)
+    assert summ.cut_batches == 0  # the default deques never reach the bound here
+    assert 0 < summ.p50_s <= summ.p99_s
+
+
+def test_uts_serve_bounded_deques(dev, oracle):
+    """Admission and work by deque memory: with deques too small for the
+    mix's unbounded high water, batches are cut and deliveries wait instead
+    of a deque overflowing, and every tree still equals its alone run."""
+    import heterosync_amd as hs
+    reqs = [(s, d, 4.0, 0.0) for s, d, _, _ in poisson_trees(91, 32, 8, 10, 0.0)]
+    reserve = 10 * 99  # deepest tree x (most children - 1)
+    _, free = dev.uts_serve(reqs, slots=16, num_blocks=64)
+    assert free.cut_batches == 0
+    assert free.deque_high_water > reserve + 256, free.deque_high_water
+    cap = max(reserve + 128, free.deque_high_water // 2)
+    resp, summ = dev.uts_serve(reqs, slots=16, num_blocks=64, deque_nodes=cap)
+    assert summ.deque_soft == cap - reserve
+    assert summ.cut_batches > 0
+    assert summ.deque_high_water <= cap < free.deque_high_water
+    for (seed, d, b0, _), r in zip(reqs, resp):
+        alone = dev.uts_geo(seed, b0, d)
+        assert (alone.nodes, alone.leaves, alone.max_depth) == (r.nodes, r.leaves, r.max_depth)
+    # a deque that cannot hold the reserve is refused before launch
+    with pytest.raises(hs.HSError) as ei:
+        dev.uts_serve(reqs, slots=4, deque_nodes=reserve)
+    assert ei.value.code == -1  # HS_EINVAL
+
+
+def test_uts_geo_depth_past_the_deque_bound_is_refused(dev):
+    import heterosync_amd as hs
+    with pytest.raises(hs.HSError) as ei:
+        dev.uts_geo(1, 4.0, 700)  # 700 x 99 records of reserve > 65536
+    assert ei.value.code == -1
