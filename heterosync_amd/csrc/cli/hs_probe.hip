@@ -9,6 +9,10 @@
 //      on the same XCD and across XCDs
 //   6. a line's return to an XCD whose L2 an earlier store left it in
 // Every wait is bounded; prints one line per measurement.
+// `hs_probe floor <WGs> <lanes> <numLdSt>` prints only the terms of the
+// synchronization floors (one JSON line): the grid barrier's data pass at
+// that footprint, dependent RMW / load / store round trips, the flag
+// hand-off on one XCD and across XCDs, and one address's RMW spacing.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -183,6 +187,41 @@ __global__ void k_return(int a, int a2, int b, int rounds, int f0, int f1, uint3
   }
 }
 
+// The data pass of a grid-barrier episode with nothing else: every WG moves
+// its own region (blockDim chains of L+1 16-word lines) `passes` times with
+// the barrier kernels' own walk (16-byte accesses, agent-scope loads, stores
+// kept in this XCD's L2, lines 1..L incremented in place), each pass drained
+// and closed by a WG barrier.  No WG waits for another, so the time per pass
+// is what the memory system needs to move one episode's data: the data term
+// of the barrier's episode floor.
+__global__ void k_l2pass(int32_t *data, int L, int passes) {
+  for (int i = 0; i < passes; ++i) {
+    wg_move<1, true>(data, (long)blockIdx.x * blockDim.x, L, true);
+    wait_mem();
+    __syncthreads();
+  }
+}
+
+// Dependent chains of one lane, `n` steps each; out[0] keeps the values live.
+//   0: agent-scope RMW (fetch_add) whose address depends on the last result
+//   1: agent-scope load whose address depends on the last loaded value
+//   2: agent-scope store followed by a drain of the memory counter
+__global__ void k_chain(int mode, int n, uint32_t *p, uint32_t *out) {
+  if (threadIdx.x != 0) return;
+  uint32_t v = 0;
+  for (int i = 0; i < n; ++i) {
+    if (mode == 0) {
+      v = add_rlx(p + (v >> 30), 1u);  // v < 2^30: always p[0], but not known to the compiler
+    } else if (mode == 1) {
+      v = ld_rlx(p + 64 * ((v + i) & 7));  // p[64 k] holds 0
+    } else {
+      st_rlx(p + 64 * (i & 7) + 32, (uint32_t)i);
+      wait_mem();
+    }
+  }
+  out[0] = v;
+}
+
 // Plain streaming copy of known size: the calibration reference for the
 // FETCH_SIZE / WRITE_SIZE hardware counters.
 __global__ void k_copy(const int4 *__restrict__ src, int4 *__restrict__ dst, long n) {
@@ -210,7 +249,60 @@ float time_ms(const std::function<void()> &f) {
 
 }  // namespace
 
+// hs_probe floor <WGs> <lanes> <numLdSt>: the measured terms of the
+// synchronization floors, one JSON line.
+int floor_terms(int nwg, int lanes, int L) {
+  if (nwg < 1 || lanes < 64 || lanes % 64 || lanes > 1024 || L < 1) return 1;
+  int32_t *data;
+  uint32_t *buf, *res;
+  const long words = (long)nwg * lanes * (L + 1);
+  CHECK(hipMalloc(&data, words * 4));
+  CHECK(hipMemset(data, 0, words * 4));
+  CHECK(hipMalloc(&buf, 1 << 16));
+  CHECK(hipMalloc(&res, 4096));
+  const int passes = 200;
+  const float pass_ms = time_ms([&] { k_l2pass<<<nwg, lanes>>>(data, L, passes); });
+  double chain_us[3];
+  for (int mode = 0; mode < 3; ++mode) {
+    const int n = 4000;
+    CHECK(hipMemset(buf, 0, 1 << 16));
+    const float ms = time_ms([&] { k_chain<<<1, 64>>>(mode, n, buf, res); });
+    chain_us[mode] = ms * 1000 / n;
+  }
+  double hop_us[2];
+  for (int k = 0; k < 2; ++k) {  // WG 8 shares WG 0's XCD, WG 1 does not
+    const int rounds = 4000, other = k == 0 ? 8 : 1;
+    CHECK(hipMemset(res, 0, 64));
+    const float ms = time_ms([&] {
+      CHECK(hipMemset(buf, 0, 256));
+      k_pingpong<<<16, 64>>>(0, other, rounds, buf, res);
+    });
+    uint32_t hr[4];
+    CHECK(hipMemcpy(hr, res, 16, hipMemcpyDeviceToHost));
+    if (hr[2]) return 2;
+    hop_us[k] = ms * 1000 / (2 * rounds);
+  }
+  CHECK(hipMemset(buf, 0, 1 << 16));
+  const int it = 1000;
+  const float one_ms = time_ms([&] { k_atomic<<<1024, 256>>>(0, it, buf); });
+  const double bytes = (double)nwg * lanes * L * 8;  // 4 B load + 4 B store per line
+  std::printf("{\"wgs\": %d, \"lanes\": %d, \"num_ldst\": %d, \"l2_pass_us\": %.4f, "
+              "\"l2_pass_gbs\": %.1f, \"rmw_rt_us\": %.4f, \"load_rt_us\": %.4f, "
+              "\"store_rt_us\": %.4f, \"hop_same_xcd_us\": %.4f, \"hop_other_xcd_us\": %.4f, "
+              "\"same_address_rmw_ns\": %.3f}\n",
+              nwg, lanes, L, pass_ms * 1000 / passes, bytes * passes / (pass_ms * 1e6),
+              chain_us[0], chain_us[1], chain_us[2], hop_us[0], hop_us[1],
+              one_ms * 1e6 / ((double)it * 1024));
+  CHECK(hipFree(data));
+  CHECK(hipFree(buf));
+  CHECK(hipFree(res));
+  return 0;
+}
+
 int main(int argc, char **argv) {
+  if (argc >= 2 && std::string(argv[1]) == "floor")  // hs_probe floor <WGs> <lanes> <numLdSt>
+    return floor_terms(argc >= 3 ? std::atoi(argv[2]) : 1024, argc >= 4 ? std::atoi(argv[3]) : 256,
+                       argc >= 5 ? std::atoi(argv[4]) : 10);
   if (argc >= 2 && std::string(argv[1]) == "copy") {  // hs_probe copy <MiB>
     const long mib = argc >= 3 ? std::atol(argv[2]) : 256;
     const long n = mib * (1L << 20) / 16;

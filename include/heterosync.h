@@ -208,7 +208,8 @@ int hs_uts_ref_run(hs_ctx *ctx, int num_nodes, const uint32_t *value,
  * num_blocks 0 = 8 per CU, or every WG that can be resident (four waves
  * per SIMD) for trees expected above 2.5e8 nodes.  Deques hold 2^16
  * records; HS_EINVAL when gen_mx * (max_children - 1) exceeds that (the
- * deque bound, see hs_uts_serve: gen_mx <= 661 at b0 = 4). */
+ * deque bound, see hs_uts_serve: gen_mx <= 689 at b0 = 4, whose nodes
+ * have at most 96 children). */
 int hs_uts_geo_run(hs_ctx *ctx, int seed, double b0, int gen_mx,
                    int num_blocks, hs_uts_result *res);
 

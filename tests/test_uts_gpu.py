@@ -127,7 +127,9 @@ def test_uts_serve_bounded_deques(dev, oracle):
     of a deque overflowing, and every tree still equals its alone run."""
     import heterosync_amd as hs
     reqs = [(s, d, 4.0, 0.0) for s, d, _, _ in poisson_trees(91, 32, 8, 10, 0.0)]
-    reserve = 10 * 99  # deepest tree x (most children - 1)
+    most = most_children(oracle, 4.0)  # most children a node can have
+    assert most == 96
+    reserve = max(d for _, d, _, _ in reqs) * (most - 1)  # deepest tree x (most - 1)
     _, free = dev.uts_serve(reqs, slots=16, num_blocks=64)
     assert free.cut_batches == 0
     assert free.deque_high_water > reserve + 256, free.deque_high_water
@@ -145,8 +147,24 @@ def test_uts_serve_bounded_deques(dev, oracle):
     assert ei.value.code == -1  # HS_EINVAL
 
 
-def test_uts_geo_depth_past_the_deque_bound_is_refused(dev):
+def most_children(oracle, b0):
+    thr = np.zeros(128, np.uint32)
+    return oracle.hso_uts_geo_thresholds(b0, thr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                         128)
+
+
+def test_uts_geo_depth_past_the_deque_bound_is_refused(dev, oracle):
+    """A depth limit whose reserve (depth x (most children - 1)) exceeds the
+    65536-record deques is refused; one at the limit runs (a subcritical
+    b0, so the tree stays small) and matches the oracle."""
     import heterosync_amd as hs
     with pytest.raises(hs.HSError) as ei:
-        dev.uts_geo(1, 4.0, 700)  # 700 x 99 records of reserve > 65536
+        dev.uts_geo(1, 4.0, 690)  # 690 x 95 records of reserve > 65536
     assert ei.value.code == -1
+    b0 = 0.9  # near-critical: seeds 1612 and 2003 reach depths 45 and 41
+    d = (65536 - 1) // (most_children(oracle, b0) - 1)
+    with pytest.raises(hs.HSError):
+        dev.uts_geo(3, b0, d + 1)
+    for seed in (3, 1612, 2003):
+        r = dev.uts_geo(seed, b0, d)
+        assert (r.nodes, r.leaves, r.max_depth) == geo_count(oracle, seed, b0, d)
