@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--only", default="",
                     help="comma list of extras to run (barriers,c1,c3,c5); default all")
+    ap.add_argument("--no-probe", action="store_true",
+                    help="skip the floor probe (a child process); the roofline then has no "
+                         "floor (for runs under a profiler)")
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
                     help="length of each CPU-baseline sample (about this much wall time)")
     return ap.parse_args()
@@ -450,12 +453,14 @@ def main():
     only = set(filter(None, a.only.split(","))) or {"barriers", "c1", "c3", "c5"}
     # the probe runs first, in its own process, before this one opens the device
     ncu_guess = 256
-    floor_terms = probe_floor(4 * ncu_guess, a.wg, a.num_ldst)
+    floor_terms = None if a.no_probe else probe_floor(4 * ncu_guess, a.wg, a.num_ldst)
     dev = hs.Device(0)
     ncu = dev.info.num_cu
     nw = 4 * ncu  # full co-residency: 4 WGs per CU
-    if nw != floor_terms["wgs"]:
+    if floor_terms and nw != floor_terms["wgs"]:
+        dev.close()
         floor_terms = probe_floor(nw, a.wg, a.num_ldst)
+        dev = hs.Device(0)
     prim = hs.prim_id(a.prim)
     reps = 10
     scope = 1 if a.strict else 0
@@ -466,7 +471,7 @@ def main():
     value = episodes / (roi_ms / 1e3)
     avg_launch_ms = kern_ms / (reps * a.steps)
     ep_us = avg_launch_ms * 1e3 / a.iters
-    floor_us = barrier_floor_us(a.prim, floor_terms)
+    floor_us = barrier_floor_us(a.prim, floor_terms) if floor_terms else None
     # HBM view: algorithmic bytes per launch = episodes/launch x WGs x lanes x
     # numLdSt x (4 B load + 4 B store), over the average launch duration
     bytes_per_episode = nw * a.wg * a.num_ldst * 8
@@ -498,14 +503,14 @@ def main():
         "roofline": {
             "bound": "sync-latency",
             "achieved": round(1e6 / ep_us, 1),
-            "peak": round(1e6 / floor_us, 1),
+            "peak": round(1e6 / floor_us, 1) if floor_us else None,
             "unit": "episodes/s",
-            "frac": round(floor_us / ep_us, 4),
+            "frac": round(floor_us / ep_us, 4) if floor_us else None,
             "traffic": traffic,
             "kernel": f"k_barrier ({a.prim})",
             "avg_launch_ms": round(avg_launch_ms, 4),
             "episode_us": round(ep_us, 4),
-            "floor_us": round(floor_us, 4),
+            "floor_us": round(floor_us, 4) if floor_us else None,
             "floor": ("one episode's data pass through the L2s at this footprint "
                       "(hs_probe floor, the kernel's own walk with no barrier) + 2 dependent "
                       "RMW round trips + 1 release hand-off across XCDs"),
@@ -518,6 +523,8 @@ def main():
         },
     }
     if not a.no_extras:
+        if floor_terms is None:
+            raise SystemExit("bench.py: the extras need the floor probe (drop --no-probe)")
         extras = {}
         if "barriers" in only:
             extras["barriers"] = extra_barriers(dev, hs, a, nw, floor_terms)
