@@ -4,6 +4,8 @@ Outputs (git-ignored, shipped with the tree):
   heterosync_amd/libheterosync_amd.so   the C-ABI library (include/heterosync.h)
   heterosync_amd/bin/allSyncPrims-1kernel, seqlocks, flags, refCounter,
                      splitCounter, uts          the drop-in command lines
+  tests/device_api/libhs_device_api_test.so    user kernels on the public
+                                               device API (GPU tests)
 
 Usage:  python -m heterosync_amd.build   (or heterosync_amd.build.build())
 """
@@ -38,6 +40,14 @@ LIB_SOURCES = [
     "hs_uts.hip",
     "hs_batch.hip",
 ]
+
+# Test extensions: kernels written against the public device API
+# (include/heterosync_device.h), loaded by the GPU tests.
+ROOT = os.path.dirname(HERE)
+TEST_LIBS = {
+    os.path.join(ROOT, "tests", "device_api", "libhs_device_api_test.so"):
+        os.path.join(ROOT, "tests", "device_api", "user_kernels.hip"),
+}
 
 CLIS = {
     "allSyncPrims-1kernel": "cli/all_sync_prims.cpp",
@@ -84,6 +94,10 @@ def build(verbose: bool = False) -> str:
         else:
             _run([HIPCC, "-O2", "-std=c++17", path, "-o", out, f"-L{HERE}",
                   "-lheterosync_amd", "-Wl,-rpath,$ORIGIN/.."])
+    for out, src in TEST_LIBS.items():
+        if os.path.exists(src):
+            _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{os.path.join(ROOT, 'include')}", src,
+                  "-o", out])
     if verbose:
         print("built", LIB)
     return LIB

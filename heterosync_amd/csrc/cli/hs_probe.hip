@@ -21,7 +21,7 @@
 #include <string>
 #include <vector>
 
-#include "../hs_syncprims_dev.h"
+#include "../../../include/hs_detail/hs_syncprims_dev.h"
 
 using namespace hs;
 

@@ -15,7 +15,7 @@
 #include <vector>
 
 #include "hs_host.h"
-#include "hs_syncprims_dev.h"
+#include "../../include/hs_detail/hs_syncprims_dev.h"
 
 using namespace hs;
 

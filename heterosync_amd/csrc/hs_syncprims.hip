@@ -18,7 +18,7 @@
 #include <vector>
 
 #include "hs_host.h"
-#include "hs_syncprims_dev.h"
+#include "../../include/hs_detail/hs_syncprims_dev.h"
 
 using namespace hs;
 
@@ -41,6 +41,7 @@ struct SPArgs {
   uint64_t ticks;
   uint32_t *xcd_map;  // per launch: physical XCD of each logical XCD (b % nxcd)
   uint32_t *flat;     // sense-reversing family: main block, local block
+  int inc_bar;        // inc-mode data pass: barrier between loads and stores
 };
 
 // AGENT: critical-section data through agent-scope accesses, drained by
@@ -204,7 +205,7 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
   __syncthreads();
   if (!s_ok) return;
   for (int i = 0; i < a.iters; ++i) {
-    wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L, dp.keep_first);
+    wg_pass<MODE, XCD>(a.inc_bar, a.storage, (long)cur * blockDim.x, a.L, dp.keep_first);
     if (LX && a.per_cu > 1) {
       bool ok;
       if (FLAG) {
@@ -221,7 +222,7 @@ __global__ void __launch_bounds__(1024) k_barrier(SPArgs a) {
       }
       if (!ok) return;
       cur = (cur + nab) % grid;  // HM:355-358
-      wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L, dp.keep_second);
+      wg_pass<MODE, XCD>(a.inc_bar, a.storage, (long)cur * blockDim.x, a.L, dp.keep_second);
     }
     bool ok;
     if (FLAG) {
@@ -305,11 +306,11 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
   };
   for (int i = 0; i < a.iters; ++i) {
     epoch = a.e_global0 + i + 1;
-    wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L, dp.keep_first);
+    wg_pass<MODE, XCD>(a.inc_bar, a.storage, (long)cur * blockDim.x, a.L, dp.keep_first);
     if (lx) {
       if (!barrier(true)) return;
       if (VAR != 3) cur = (cur + nab) % grid;
-      wg_move<MODE, XCD>(a.storage, (long)cur * blockDim.x, a.L, dp.keep_second);
+      wg_pass<MODE, XCD>(a.inc_bar, a.storage, (long)cur * blockDim.x, a.L, dp.keep_second);
     }
     if (!barrier(false)) return;
     cur = (cur + 1) % grid;
@@ -565,6 +566,7 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   a.flat = d_flat;
   a.nxcd = nxcd;
   a.L = L;
+  a.inc_bar = cfg->mode == 1 && inc_band_barrier(wg, L);
   a.iters = iters;
   a.num_cu = ncu;
   a.nab = nab;

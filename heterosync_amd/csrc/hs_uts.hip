@@ -44,7 +44,7 @@
 #include <vector>
 
 #include "hs_host.h"
-#include "hs_device.h"
+#include "../../include/hs_detail/hs_device.h"
 
 using namespace hs;
 

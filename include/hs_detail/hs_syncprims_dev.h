@@ -164,7 +164,7 @@ __device__ __forceinline__ void wg_access(int32_t *s, long tid0, int L) {
 typedef uint32_t hs_v4u __attribute__((ext_vector_type(4)));
 constexpr int kRsrcWord3 = 0x00020000;  // raw buffer descriptor, gfx9 family
 
-template <int MODE, bool SC1 = false, int BAND = 16>
+template <int MODE, bool SC1 = false, int BAND = 16, bool INC_BAR = false>
 __device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L, bool keep_l2 = false) {
   static_assert(BAND % 16 == 0 && BAND >= 16 && BAND <= 64, "band of 16..64 lines");
   constexpr int kItems = BAND / 4;  // 16-byte pieces per thread per band
@@ -190,7 +190,10 @@ __device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L, bool keep_
         v[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, src, 0, kAux);
       }
     }
-    if (MODE == 0) {  // other lanes store into lines this band still reads
+    // Copy mode: other lanes store into lines this band still reads.  Inc
+    // mode (INC_BAR) takes the same barrier where two waves touch one
+    // 128-byte line (see inc_band_barrier).
+    if (MODE == 0 || INC_BAR) {
       wait_mem();
       __syncthreads();
     }
@@ -206,6 +209,32 @@ __device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L, bool keep_
           __builtin_amdgcn_raw_buffer_store_b128(MODE == 0 ? v[k] : v[k] + 1u, rs, dst[k], 0, 0);
     }
   }
+}
+
+// Whether an inc-mode band walk of WGs of `wg` threads should drain and
+// barrier between its loads and stores (wg_move's INC_BAR).  Each wave holds
+// 16 consecutive lines of a band's chain-major order, so a wave boundary
+// falls inside a chain unless the band's lines per chain divide 16; one
+// wave's stores then interleave with another wave's loads of a shared
+// 128-byte line.  Measured at 1024 x 256 (XCD-aware atomicTreeBarrUniq, one
+// box): numLdSt 10, 3.90 us per episode without the barrier and 3.41 with it
+// (copy mode 3.34); numLdSt 1 and 4, 2.64 and 2.75 without it, 2.72 and
+// 2.84 with it.  The choice is per launch and compile-time inside the walk:
+// a run-time test between the loads and the stores measured slower than
+// either form.
+__host__ __device__ inline bool inc_band_barrier(int wg, int L, int band = 16) {
+  if (wg <= 64 || L <= 0) return false;
+  const int top = L < band ? L : band, rest = L % band;
+  return band % top != 0 || (L > band && rest != 0 && band % rest != 0);
+}
+
+// The data pass of a barrier episode: wg_move, with the inc-mode band
+// barrier chosen per launch (a uniform kernel argument).
+template <int MODE, bool SC1>
+__device__ __forceinline__ void wg_pass(bool inc_bar, int32_t *s, long tid0, int L,
+                                        bool keep_l2 = false) {
+  if (MODE == 1 && inc_bar) wg_move<1, SC1, 16, true>(s, tid0, L, keep_l2);
+  else wg_move<MODE, SC1, 16, false>(s, tid0, L, keep_l2);
 }
 
 // Semaphore reader body (accessData_semRd, HM:89-100): L loads, summed
