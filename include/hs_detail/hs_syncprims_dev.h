@@ -216,16 +216,17 @@ __device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L, bool keep_
 // 16 consecutive lines of a band's chain-major order, so a wave boundary
 // falls inside a chain unless the band's lines per chain divide 16; one
 // wave's stores then interleave with another wave's loads of a shared
-// 128-byte line.  Measured at 1024 x 256 (XCD-aware atomicTreeBarrUniq, one
-// box): numLdSt 10, 3.90 us per episode without the barrier and 3.41 with it
-// (copy mode 3.34); numLdSt 1 and 4, 2.64 and 2.75 without it, 2.72 and
-// 2.84 with it.  The choice is per launch and compile-time inside the walk:
-// a run-time test between the loads and the stores measured slower than
-// either form.
+// 128-byte line; a walk of several bands also keeps the next band's loads
+// behind the stores of the one above.  Measured at 1024 x 256 (XCD-aware
+// atomicTreeBarrUniq, one box): numLdSt 10, 3.90 us per episode without the
+// barrier and 3.41 with it (copy mode 3.34); numLdSt 100, 34.0 without and
+// 32.8 with it (copy 32.6-32.8); numLdSt 1 and 4, 2.64 and 2.75 without it,
+// 2.72 and 2.84 with it.  The choice is per launch and compile-time inside
+// the walk: a run-time test between the loads and the stores measured
+// slower than either form.
 __host__ __device__ inline bool inc_band_barrier(int wg, int L, int band = 16) {
   if (wg <= 64 || L <= 0) return false;
-  const int top = L < band ? L : band, rest = L % band;
-  return band % top != 0 || (L > band && rest != 0 && band % rest != 0);
+  return L > band || band % L != 0;
 }
 
 // The data pass of a barrier episode: wg_move, with the inc-mode band
