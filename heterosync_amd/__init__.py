@@ -21,14 +21,14 @@ BIN_DIR = os.path.join(HERE, "bin")
 HS_OK = 0
 ERRORS = {
     -1: "HS_EINVAL", -2: "HS_EHIP", -3: "HS_ENOMEM", -4: "HS_EOCCUPANCY",
-    -5: "HS_ETIMEOUT", -6: "HS_ENODEV", -7: "HS_EUNSUPPORTED",
+    -5: "HS_ETIMEOUT", -6: "HS_ENODEV", -7: "HS_EUNSUPPORTED", -8: "HS_ESCRATCH",
 }
 
 # Every symbol include/heterosync.h declares.
 EXPORTED = [
     "hs_strerror", "hs_open", "hs_close", "hs_device_info", "hs_prim_id",
     "hs_prim_name", "hs_prim_supported", "hs_storage_words", "hs_syncprim_expected",
-    "hs_syncprim_run",
+    "hs_syncprim_run", "hs_syncprim_run_noalloc",
     "hs_seqlock_run", "hs_flags_run", "hs_refcounter_run", "hs_splitcounter_run",
     "hs_uts_ref_run", "hs_uts_geo_run", "hs_batch_run", "hs_batch_autotune",
     "hs_uts_serve",
@@ -140,6 +140,9 @@ def _lib():
                                                                   ctypes.c_long]
         lib.hs_syncprim_run.argtypes = [ctypes.c_void_p, P(SyncPrimCfg), P(ctypes.c_int32),
                                         ctypes.c_long, P(SyncPrimResult)]
+        lib.hs_syncprim_run_noalloc.argtypes = [ctypes.c_void_p, P(SyncPrimCfg),
+                                                ctypes.c_void_p, ctypes.c_long,
+                                                P(SyncPrimResult)]
         lib.hs_seqlock_run.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [
             P(ctypes.c_uint32), ctypes.c_int, P(ctypes.c_int32), P(ctypes.c_uint64),
             P(RelaxedResult)]

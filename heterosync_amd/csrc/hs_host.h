@@ -18,6 +18,10 @@ struct hs_ctx {
   double wall_hz = 0;  // wall_clock64() ticks per second
   char arch[64] = {0};
   hipStream_t stream = nullptr;
+  // the no-allocation entry point's per-device scratch array (device
+  // address from hipGetSymbolAddress) and its size in words
+  uint32_t *scratch = nullptr;
+  size_t scratch_words = 0;
   // engine kept between UTS calls, released by hs_close
   void *uts_cache = nullptr;
   void (*uts_cache_free)(void *) = nullptr;
@@ -85,6 +89,10 @@ inline uint64_t timeout_ticks(const hs_ctx *c, double s) {
   if (s <= 0) s = 20.0;
   return (uint64_t)(s * c->wall_hz);
 }
+
+// Looks up the device address of the no-allocation scratch array for the
+// context's device (current when called).
+int scratch_init(hs_ctx *c);
 
 // Max co-resident WGs of `kernel` at `wg` threads per WG on this device.
 int coresident_wgs(const hs_ctx *c, const void *kernel, int wg, int *out);

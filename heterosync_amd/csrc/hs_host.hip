@@ -87,6 +87,7 @@ const char *hs_strerror(int code) {
     case HS_ETIMEOUT: return "device watchdog fired (wait exceeded timeout)";
     case HS_ENODEV: return "no HIP device";
     case HS_EUNSUPPORTED: return "primitive not part of this build";
+    case HS_ESCRATCH: return "workload needs more than the no-allocation scratch (4 MiB)";
     default: return "unknown error";
   }
 }
@@ -113,6 +114,11 @@ int hs_open(int device, hs_ctx **out) {
   c->wall_hz = (wall_khz > 0 ? wall_khz : 100000) * 1000.0;
   std::strncpy(c->arch, prop.gcnArchName, sizeof(c->arch) - 1);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return HS_EHIP;
+  }
+  if (hs::scratch_init(c) != HS_OK) {
+    (void)hipStreamDestroy(c->stream);
     delete c;
     return HS_EHIP;
   }

@@ -13,8 +13,10 @@
 //                               without local exchange
 //                               (main.hip.cpp:104-149, 315-373, 542-643)
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "hs_host.h"
@@ -318,6 +320,12 @@ __global__ void __launch_bounds__(1024) k_srb(SPArgs a) {
 }
 
 // Initialises lock blocks and (sleep mutex) ring slots.
+__global__ void k_fill(uint32_t *p, size_t n, uint32_t v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 __global__ void k_init_locks(uint32_t *locks, int nlocks, uint32_t sem_max,
                              uint32_t *ring, long ring_len, int stride) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -403,10 +411,61 @@ int num_xcds(const hs_ctx *c) {
   return c->num_cu >= 64 && c->num_cu % 32 == 0 ? c->num_cu / 32 : 1;
 }
 
+// The no-allocation form's buffers: the caller's device storage, and the
+// per-device scratch array (see hs_syncprim_run_noalloc).
+struct NoAlloc {
+  int32_t *d_storage;
+  uint32_t *scratch;
+  size_t scratch_words;
+};
+
 int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
-             hs_syncprim_result *res, bool xcd_mode, bool *topology_fail);
+             hs_syncprim_result *res, bool xcd_mode, bool *topology_fail,
+             const NoAlloc *na = nullptr);
 
 }  // namespace
+
+// The no-allocation entry point's state: lock blocks, queue ring, barrier
+// words, flags, control words and placement map of one run, carved from
+// this array.  Each GPU the library runs on holds its own copy (4 MiB) for
+// the life of the process; hs_open looks up its device address once per
+// context, with that context's device current.
+constexpr size_t kScratchWords = (size_t)1 << 20;
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+namespace hs {
+int scratch_init(hs_ctx *c) {
+  void *ptr = nullptr;
+  HS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  c->scratch = static_cast<uint32_t *>(ptr);
+  c->scratch_words = kScratchWords;
+  return HS_OK;
+}
+}  // namespace hs
+
+extern "C" int hs_syncprim_run_noalloc(hs_ctx *c, const hs_syncprim_cfg *cfg,
+                                       int32_t *d_storage, long n, hs_syncprim_result *res) {
+  if (!c || !cfg || !d_storage || !res) return HS_EINVAL;
+  if (!c->scratch) return HS_EHIP;
+  const NoAlloc na{d_storage, c->scratch, c->scratch_words};
+  const int p = cfg->prim, ncu = c->num_cu;
+  const int per = cfg->num_wgs > 0 ? (cfg->num_wgs + std::min(cfg->num_wgs, ncu) - 1) /
+                                         std::min(cfg->num_wgs, ncu)
+                                   : 1;
+  const bool xcd_form = (p >= 0 && p < 4) || ((p == 37 || p == 38) && per > 4);
+  const bool srb_flat = p >= 37 && p <= 42 && !xcd_form;
+  const int nx = num_xcds(c);
+  if (cfg->sync_scope == 0 && (is_mutex(p) || is_sem(p) || (srb_flat && nx == 1)))
+    return run_once(c, cfg, nullptr, n, res, true, nullptr, &na);
+  const bool xcd = cfg->sync_scope == 0 && (xcd_form || srb_flat) && nx > 1;
+  if (!xcd) return run_once(c, cfg, nullptr, n, res, false, nullptr, &na);
+  // The placement check cannot fall back here: the storage has no saved
+  // copy to restart from.  sync_scope_used 2 reports it.
+  bool topo = false;
+  const int rc = run_once(c, cfg, nullptr, n, res, true, &topo, &na);
+  if (rc == HS_ETIMEOUT && topo) res->sync_scope_used = 2;
+  return rc;
+}
 
 extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
                                int32_t *storage, long n, hs_syncprim_result *res) {
@@ -442,7 +501,7 @@ extern "C" int hs_syncprim_run(hs_ctx *c, const hs_syncprim_cfg *cfg,
 namespace {
 
 int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
-             hs_syncprim_result *res, bool xcd_mode, bool *topology_fail) {
+             hs_syncprim_result *res, bool xcd_mode, bool *topology_fail, const NoAlloc *na) {
   std::memset(res, 0, sizeof(*res));
   const int p = cfg->prim;
   if (hs_prim_name(p) == nullptr) return HS_EINVAL;
@@ -484,17 +543,42 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
     tickets *= (long)iters * reps;
     ring_len = tickets + 1;
     // each slot on its own 128-byte line while the rings stay under 128 MiB
-    ring_stride = ((long)nlocks * ring_len <= (1L << 20)) ? HS_LINE_WORDS : 1;
+    // (the no-allocation form: while they take at most a quarter of its
+    // scratch)
+    const long line_limit = na ? (long)(na->scratch_words / 4 / HS_LINE_WORDS) : (1L << 20);
+    ring_stride = ((long)nlocks * ring_len <= line_limit) ? HS_LINE_WORDS : 1;
   }
   DevArena arena;
-  int32_t *d_storage;
+  size_t carved = 0;  // words of the scratch handed out (no-allocation form)
+  // Device buffers: allocated, or carved from the scratch at 256-byte
+  // boundaries (HS_ESCRATCH once it is used up).
+  auto take = [&](auto **out, size_t count) -> int {
+    if (!na) return arena.alloc(out, count);
+    const size_t words = (count * sizeof(**out) + 3) / 4;
+    if (carved + words > na->scratch_words) return HS_ESCRATCH;
+    *out = reinterpret_cast<std::remove_reference_t<decltype(*out)>>(na->scratch + carved);
+    carved += (words + 63) & ~(size_t)63;
+    return HS_OK;
+  };
+  // Fills on the stream: memsets, or a fill kernel in the no-allocation form
+  // (which callers may run where memset nodes are unwelcome).
+  auto fill = [&](uint32_t *p, size_t words, uint32_t v) -> int {
+    if (!na) {
+      HS_HIP(hipMemsetAsync(p, v == 0 ? 0 : 0xff, words * 4, c->stream));
+      return HS_OK;
+    }
+    const int blocks = (int)std::min<size_t>((words + 255) / 256, 1024);
+    k_fill<<<blocks > 0 ? blocks : 1, 256, 0, c->stream>>>(p, words, v);
+    HS_HIP(hipGetLastError());
+    return HS_OK;
+  };
+  int32_t *d_storage = na ? na->d_storage : nullptr;
   uint32_t *d_ctl, *d_locks, *d_ring = nullptr, *d_bar, *d_flags;
   int rc;
-  if ((rc = arena.alloc(&d_storage, n))) return rc;
-  if ((rc = arena.alloc(&d_ctl, 64))) return rc;
-  if ((rc = arena.alloc(&d_locks, (size_t)nlocks * kLockWords))) return rc;
-  if (sleep_mx && (rc = arena.alloc(&d_ring, (size_t)nlocks * ring_len * ring_stride)))
-    return rc;
+  if (!na && (rc = arena.alloc(&d_storage, n))) return rc;
+  if ((rc = take(&d_ctl, 64))) return rc;
+  if ((rc = take(&d_locks, (size_t)nlocks * kLockWords))) return rc;
+  if (sleep_mx && (rc = take(&d_ring, (size_t)nlocks * ring_len * ring_stride))) return rc;
   // barrier words, kBarWords blocks: [global][XCD-aware sub-counters:
   // nxcd x sub_max][groups: nab][release words: nxcd][episode count line]
   int sub_max = 0, n_sub = 0;  // XCD-aware tree: sub-counters per XCD, in all
@@ -508,7 +592,7 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   const size_t o_sub = kBarWords, o_grp = o_sub + (size_t)kBarWords * nxcd * sub_max;
   const size_t o_rel = o_grp + (size_t)kBarWords * nab, o_ep = o_rel + (size_t)kBarWords * nxcd;
   const size_t bar_words = o_ep + HS_LINE_WORDS;
-  if ((rc = arena.alloc(&d_bar, bar_words))) return rc;
+  if ((rc = take(&d_bar, bar_words))) return rc;
   // strict: packed flag arrays; XCD: one line per flag (kFlagStride)
   // flags: strict [gin: nab][gout: nab][lin, lout: nab x per], packed;
   // XCD-aware [gout: nxcd][din: nxcd x sub_max][sin: nxcd x sub_max x
@@ -518,16 +602,17 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   const size_t flag_words = (xcd_mode ? x_flags * kFlagStride
                                       : 2 * (size_t)nab + 2 * (size_t)nab * per) +
                             4 * HS_LINE_WORDS;
-  if ((rc = arena.alloc(&d_flags, flag_words))) return rc;
+  if ((rc = take(&d_flags, flag_words))) return rc;
   uint32_t *d_xmap, *d_flat;
-  if ((rc = arena.alloc(&d_xmap, (size_t)reps * nxcd))) return rc;
-  if ((rc = arena.alloc(&d_flat, 2 * (size_t)kBarWords))) return rc;
-  HS_HIP(hipMemsetAsync(d_flat, 0, 2 * kBarWords * 4, c->stream));
-  HS_HIP(hipMemsetAsync(d_xmap, 0xff, (size_t)reps * nxcd * 4, c->stream));
-  HS_HIP(hipMemsetAsync(d_ctl, 0, 64 * sizeof(uint32_t), c->stream));
-  HS_HIP(hipMemsetAsync(d_locks, 0, (size_t)nlocks * kLockWords * 4, c->stream));
-  HS_HIP(hipMemsetAsync(d_bar, 0, bar_words * 4, c->stream));
-  HS_HIP(hipMemsetAsync(d_flags, 0, flag_words * 4, c->stream));
+  if ((rc = take(&d_xmap, (size_t)reps * nxcd))) return rc;
+  if ((rc = take(&d_flat, 2 * (size_t)kBarWords))) return rc;
+  // the scratch keeps the last call's contents: every word used is cleared
+  if ((rc = fill(d_flat, 2 * kBarWords, 0))) return rc;
+  if ((rc = fill(d_xmap, (size_t)reps * nxcd, 0xffffffffu))) return rc;
+  if ((rc = fill(d_ctl, 64, 0))) return rc;
+  if ((rc = fill(d_locks, (size_t)nlocks * kLockWords, 0))) return rc;
+  if ((rc = fill(d_bar, bar_words, 0))) return rc;
+  if ((rc = fill(d_flags, flag_words, 0))) return rc;
   const uint32_t sem_max = sem_count_of(p) ? sem_count_of(p) + 1 : 0;  // count+1, HSS:15
   if (sem_max || d_ring) {
     long work = std::max<long>(nlocks, nlocks * ring_len);
@@ -536,8 +621,9 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
                                                 ring_len, ring_stride);
     HS_HIP(hipGetLastError());
   }
-  HS_HIP(hipMemcpyAsync(d_storage, storage, n * sizeof(int32_t), hipMemcpyHostToDevice,
-                        c->stream));
+  if (!na)
+    HS_HIP(hipMemcpyAsync(d_storage, storage, n * sizeof(int32_t), hipMemcpyHostToDevice,
+                          c->stream));
   HS_HIP(hipStreamSynchronize(c->stream));
 
   SPArgs a;
@@ -582,13 +668,29 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
 
   // ---- region of interest (main.hip.cpp:2354-2511) -----------------------
   if ((rc = load_kernel((const void *)kfn))) return rc;  // not inside the ROI
+  if (na) {  // no events: the host clock around the launches and their syncs
+    HS_HIP(hipStreamSynchronize(c->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int r = 0; r < reps; ++r) {
+      a.xcd_map = d_xmap + (size_t)r * nxcd;
+      a.e_global0 = (uint32_t)r * iters;
+      a.e_local0 = (uint32_t)r * iters * local_per_iter;
+      hipLaunchKernelGGL(kfn, dim3(nw), dim3(wg), 0, c->stream, a);
+      HS_HIP(hipGetLastError());
+      HS_HIP(hipStreamSynchronize(c->stream));
+    }
+    res->roi_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    res->kernel_ms = res->roi_ms;
+  }
   EventSet ev;  // [0] ROI start, [1] ROI end, then a pair per launch
-  if ((rc = ev.create(2 + 2 * (size_t)reps))) return rc;
-  const hipEvent_t ev0 = ev[0], ev1 = ev[1];
+  if (!na && (rc = ev.create(2 + 2 * (size_t)reps))) return rc;
+  const hipEvent_t ev0 = na ? nullptr : ev[0], ev1 = na ? nullptr : ev[1];
   auto kev = [&](int i) { return ev[2 + i]; };
+  int launch_rc = HS_OK;
+  if (!na) {
   HS_HIP(hipDeviceSynchronize());
   HS_HIP(hipEventRecord(ev0, c->stream));
-  int launch_rc = HS_OK;
   for (int r = 0; r < reps; ++r) {
     a.xcd_map = d_xmap + (size_t)r * nxcd;
     a.e_global0 = (uint32_t)r * iters;
@@ -611,10 +713,11 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
     kms += k;
   }
   res->kernel_ms = kms;
+  }
   if (launch_rc) return launch_rc;
 
   // ---- results and idle-state invariants ----------------------------------
-  HS_HIP(hipMemcpy(storage, d_storage, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (!na) HS_HIP(hipMemcpy(storage, d_storage, n * sizeof(int32_t), hipMemcpyDeviceToHost));
   uint32_t ctlw[64];
   HS_HIP(hipMemcpy(ctlw, d_ctl, sizeof ctlw, hipMemcpyDeviceToHost));
   const uint32_t abort_w = ctlw[0];

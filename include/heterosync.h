@@ -47,9 +47,11 @@ enum {
   HS_EOCCUPANCY = -4,  /* workgroups cannot all be co-resident */
   HS_ETIMEOUT = -5,    /* device watchdog fired: a wait exceeded timeout_s */
   HS_ENODEV = -6,      /* no HIP device */
-  HS_EUNSUPPORTED = -7 /* primitive id without a kernel in this build (every
+  HS_EUNSUPPORTED = -7, /* primitive id without a kernel in this build (every
                           id of the reference table has one; kept for ABI
                           stability) */
+  HS_ESCRATCH = -8     /* hs_syncprim_run_noalloc: the run's state does not
+                          fit the 4 MiB scratch */
 };
 
 const char *hs_strerror(int code);
@@ -119,6 +121,28 @@ typedef struct {
  * HS_EOCCUPANCY otherwise. */
 int hs_syncprim_run(hs_ctx *ctx, const hs_syncprim_cfg *cfg, int32_t *storage,
                     long n_words, hs_syncprim_result *res);
+
+/* hs_syncprim_run without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * d_storage is the caller's DEVICE array of n_words =
+ * hs_storage_words(...) words, read and written in place (initialise it to
+ * storage[i] = i for the reference's workload).  Every other word of state
+ * (lock blocks, the queue lock's ring, barrier counters and flags, control
+ * words) is carved from one static 4 MiB device array per GPU, whose
+ * address hs_open looks up; the call makes no hipMalloc, hipFree or event
+ * and clears what it uses with a fill kernel.  Runs whose state exceeds
+ * 4 MiB (long queue-lock rings, more than about 16 WGs per CU) return
+ * HS_ESCRATCH.  All calls on one GPU share that array and keep the last
+ * call's contents: calls that overlap on one GPU (other host threads or
+ * contexts on the same device) race.  roi_ms and kernel_ms are host-clock
+ * times of the launches and their synchronizes.  If the XCD-aware barriers'
+ * placement check fails (they then run in the strict form in
+ * hs_syncprim_run), this call cannot restart and returns HS_ETIMEOUT with
+ * sync_scope_used = 2. */
+int hs_syncprim_run_noalloc(hs_ctx *ctx, const hs_syncprim_cfg *cfg,
+                            int32_t *d_storage, long n_words,
+                            hs_syncprim_result *res);
 
 /* The storage a correct run leaves: the reference binary's own self-check
  * (the host golden of main.hip.cpp:2524-2702), in closed form.  Fills
