@@ -197,7 +197,21 @@ def serialized_acquires(p, nw, iters, ncu):
     return writers * iters
 
 
-def extra_c3(dev, hs):
+def batched_acquire_floor_us(p, nw, L, ncu, t):
+    """Lower bound on one exclusive acquire+release of a batched instance:
+    the release is seen by the next holder (a hand-off across XCDs), which
+    makes its CAS where the lock needs one (spin, EBO, semaphores), loads
+    its lines (at least one round trip per 64 lines: the widest worker
+    splits a chain over four waves, 16 lines per load batch) and drains its
+    stores (one store round trip) before releasing.  A semaphore writer
+    walks its group's perCU regions."""
+    regions = 1 if p <= 7 else -(-nw // min(nw, ncu))
+    cas = t["rmw_rt_us"] if p in (4, 5) or p >= 8 else 0.0
+    cs = max(1, -(-L // 64)) * t["load_rt_us"] * regions + t["store_rt_us"]
+    return t["hop_other_xcd_us"] + cas + cs
+
+
+def extra_c3(dev, hs, floor_terms):
     """The batched configuration: the seed-3 mix of 64 instances in one
     persistent hipGraph-captured launch (autotuned, inc mode) against the 64
     plain launches of the same instances."""
@@ -221,11 +235,12 @@ def extra_c3(dev, hs):
         plain_ok = plain_ok and bool(np.array_equal(ps, s)) and r.state_ok == 1
     # critical-path bound: the instances run concurrently, so the launch
     # takes at least the longest chain of mutually exclusive acquires, each
-    # at least the measured plain hand-off time at that instance's numLdSt
-    hand = {L: handoff_us(dev, L) for L in sorted({L for *_, L in mix})}
-    chains = [(serialized_acquires(p, nw, it, ncu) * hand[L], i)
+    # at least the floor of one hand-off (measured round trips)
+    chains = [(serialized_acquires(p, nw, it, ncu) *
+               batched_acquire_floor_us(p, nw, L, ncu, floor_terms), i)
               for i, (p, nw, it, L) in enumerate(mix)]
     bound_us, crit = max(chains)
+    hand = {L: handoff_us(dev, L) for L in sorted({L for *_, L in mix})}
     batched = rb.acquires / (rb.roi_ms / 1e3)
     plain = plain_acq / (plain_ms / 1e3)
     cp, cnw, cit, cL = mix[crit]
@@ -241,10 +256,14 @@ def extra_c3(dev, hs):
             "critical_path": {"instance": crit, "prim": hs.prim_name(cp), "num_wgs": cnw,
                               "num_cs_iters": cit, "num_ldst": cL,
                               "serialized_acquires": serialized_acquires(cp, cnw, cit, ncu),
-                              "handoff_us_by_numldst": {str(k): round(v, 4)
-                                                        for k, v in hand.items()},
+                              "acquire_floor_us": round(batched_acquire_floor_us(
+                                  cp, cnw, cL, ncu, floor_terms), 4),
+                              "achieved_us_per_acquire": round(
+                                  rb.roi_ms * 1e3 / serialized_acquires(cp, cnw, cit, ncu), 4),
                               "bound_ms": round(bound_us / 1e3, 3),
-                              "frac_of_bound": round(bound_us / 1e3 / rb.roi_ms, 4)},
+                              "frac_of_bound": round(bound_us / 1e3 / rb.roi_ms, 4),
+                              "plain_famutex_us_per_acquire_by_numldst": {
+                                  str(k): round(v, 4) for k, v in hand.items()}},
             "parity_ok": ok and plain_ok and plain_acq == rb.acquires}
 
 
@@ -531,7 +550,7 @@ def main():
         if "c1" in only:
             extras["c1_faMutex"] = extra_c1(dev, hs, floor_terms)
         if "c3" in only:
-            extras["c3_batched"] = extra_c3(dev, hs)
+            extras["c3_batched"] = extra_c3(dev, hs, floor_terms)
         if "c5" in only:
             extras["c5_serving"] = extra_c5(dev)
         out["extras"] = extras

@@ -122,11 +122,29 @@ __device__ __forceinline__ void vwg_read(const int32_t *s, long tid0, int L,
 // 1..L, so a lost, doubled or overlapping critical section changes the
 // final storage (with a copy chain every pass past the L-th leaves the same
 // array).  S: waves per worker WG; all of them share each critical section.
-template <int MODE, int S>
+//
+// A pass over the worker's virtual WGs makes one attempt for each, in one of
+// two ways (the autotuner picks):
+//   PAR 0, serial: lane 0 attempts the virtual WGs one after another (a
+//     test-and-test-and-set, a ticket or queue-slot check, a semaphore
+//     CAS), one memory round trip per virtual WG;
+//   PAR 1, probed: lane j of the first wave first probes virtual WG j's lock
+//     state (one load per lane, all in flight together: one round trip for
+//     the pass), taking a ticket or queue slot where the lock needs one.  A
+//     ticket whose turn has come, or a queue slot holding the hand-off, is
+//     the lock: that virtual WG enters.  A free lock word or an enterable
+//     semaphore word is only a hint: lane 0 then makes the CAS.  Failed
+//     probes of EBO instances count as failed attempts for the backoff.
+// Measured on the seeded 64-instance mix (inc mode, one launch): probed
+// 1.10 s with 256-lane workers against serial 1.50 s with 64-lane ones (each
+// form's best), but probed is slower with 64-lane workers (2.5-3.3 s against
+// 1.5-2.2 s): every pass of many narrow workers loads every lock word.
+template <int MODE, int S, bool PAR>
 __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
   __shared__ int s_inst[kMaxVpw], s_b[kMaxVpw], s_iter[kMaxVpw], s_phase[kMaxVpw];
   __shared__ uint32_t s_ticket[kMaxVpw], s_skip[kMaxVpw], s_bo[kMaxVpw];
   __shared__ int s_remaining, s_enter, s_ok;
+  __shared__ unsigned long long s_cand, s_sure;
   __shared__ int32_t sink[64 * S];
   const int lane = threadIdx.x, P = gridDim.x, w = blockIdx.x;
   Guard g;
@@ -153,10 +171,80 @@ __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
     s_ok = !g.aborted();
   }
   __syncthreads();
+  // the probing lane's own virtual WG, fixed for the kernel
+  int my_p = -1;
+  bool my_writer = false;
+  uint32_t my_max = 0;
+  uint32_t *my_lk = nullptr, *my_ring = nullptr;
+  int my_stride = 0;
+  if (lane < a.vpw && s_phase[lane] != kDone) {
+    const BInst &I = a.inst[s_inst[lane]];
+    my_p = I.prim;
+    my_lk = I.lk;
+    my_ring = I.ring;
+    my_stride = I.ring_stride;
+    my_max = I.sem_max;
+    const int b = s_b[lane], grp = b % I.nab, pw = b / I.nab;
+    my_writer = my_p < 8 || pw == grp % I.per;  // main.hip.cpp:883
+  }
   while (s_remaining > 0 && s_ok) {
+    // ---- probe: one lane per virtual WG (serial: every live one is a candidate)
+    if (!PAR && lane < 64) {
+      const unsigned long long live = __ballot(lane < a.vpw && s_phase[lane] != kDone);
+      if (lane == 0) {
+        s_cand = live;
+        s_sure = 0;
+      }
+    }
+    if (PAR && lane < 64) {
+      int maybe = 0, sure = 0;
+      bool probed = false;
+      const int j = lane;
+      if (my_p >= 0 && s_phase[j] != kDone) {
+        probed = s_skip[j] == 0;
+        if (!probed) {
+          --s_skip[j];
+        } else if (my_p == 4 || my_p == 5) {  // spin / EBO: the lock word is free
+          maybe = ld_rlx(my_lk + kLkHead) == 0;
+        } else if (my_p == 7) {  // ticket: take one, then compare the turn
+          if (s_phase[j] == kIdle) {
+            s_ticket[j] = add_rlx(my_lk + kLkHead, 1);
+            s_phase[j] = kWaiting;
+          }
+          sure = ld_rlx(my_lk + kLkTail) == s_ticket[j];
+        } else if (my_p == 6) {  // queue lock: take a slot, then check it
+          if (s_phase[j] == kIdle) {
+            s_ticket[j] = add_rlx(my_lk + kLkTail, 1);
+            s_phase[j] = kWaiting;
+          }
+          sure = ld_rlx(my_ring + (long)s_ticket[j] * my_stride) == 1u;
+        } else {  // semaphore: could this WG enter, or announce a waiting writer?
+          const uint32_t st = ld_rlx(my_lk + kLkHead);
+          const uint32_t cnt = st & 0xffffu;
+          const bool waiting = (st & kSemWaitBit) != 0;
+          maybe = my_writer ? (cnt == my_max || !waiting) : (!waiting && cnt > 1);
+        }
+        // a busy probe is a failed attempt for the EBO schedules
+        if (probed && !maybe && !sure && (my_p == 5 || my_p >= 12)) {
+          s_skip[j] = s_bo[j] >> 2;
+          uint32_t nb = (my_p == 5 || my_writer) ? ((s_bo[j] << 1) + 1) & a.bo.max_q
+                                                 : s_bo[j] + 5;
+          s_bo[j] = nb > a.bo.max_q ? a.bo.max_q : nb;
+        }
+      }
+      const unsigned long long mc = __ballot(maybe || sure), ms = __ballot(sure);
+      if (lane == 0) {
+        s_cand = mc;
+        s_sure = ms;
+      }
+    }
+    __syncthreads();
+    unsigned long long cand = s_cand;
+    const unsigned long long sure = s_sure;
     int progress = 0;
-    for (int j = 0; j < a.vpw; ++j) {
-      if (s_phase[j] == kDone) continue;  // uniform (LDS, read after a barrier)
+    while (cand) {
+      const int j = (int)__builtin_ctzll(cand);
+      cand &= cand - 1;
       const BInst &I = a.inst[s_inst[j]];
       const int p = I.prim;
       const bool sem = p >= 8;
@@ -164,8 +252,16 @@ __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
       const int grp = b % I.nab, pw = b / I.nab;
       const bool writer = !sem || pw == grp % I.per;  // main.hip.cpp:883
       if (lane == 0) {
-        int enter = 0;
-        if (s_skip[j] > 0) {
+        int enter = (sure >> j) & 1;
+        if (PAR) {
+          if (!enter && (p == 4 || p == 5)) {
+            enter = cas_rlx(I.lk + kLkHead, 0, 1) == 0;
+          } else if (!enter && sem) {
+            const int r = sem_try_wait<true>(I.lk, writer, g);
+            if (r < 0) s_ok = 0;
+            enter = r > 0;
+          }
+        } else if (s_skip[j] > 0) {
           --s_skip[j];
         } else if (!sem) {
           uint32_t *lk = I.lk;
@@ -330,13 +426,19 @@ int worker_waves(const hs_batch_cfg *cfg) {
   return w == 64 ? 1 : w == 128 ? 2 : w == 256 ? 4 : 0;
 }
 
-BatchFn pick_batch(int mode, int waves) {
+template <bool PAR>
+BatchFn pick_batch_t(int mode, int waves) {
   switch (waves) {
-    case 1: return mode ? k_batch<1, 1> : k_batch<0, 1>;
-    case 2: return mode ? k_batch<1, 2> : k_batch<0, 2>;
-    case 4: return mode ? k_batch<1, 4> : k_batch<0, 4>;
+    case 1: return mode ? k_batch<1, 1, PAR> : k_batch<0, 1, PAR>;
+    case 2: return mode ? k_batch<1, 2, PAR> : k_batch<0, 2, PAR>;
+    case 4: return mode ? k_batch<1, 4, PAR> : k_batch<0, 4, PAR>;
     default: return nullptr;
   }
+}
+
+BatchFn pick_batch(const hs_batch_cfg *cfg) {
+  const int waves = worker_waves(cfg);
+  return cfg->probe ? pick_batch_t<true>(cfg->mode, waves) : pick_batch_t<false>(cfg->mode, waves);
 }
 
 // Runs `reps` persistent launches (optionally as one captured graph) and
@@ -344,7 +446,7 @@ BatchFn pick_batch(int mode, int waves) {
 int batch_launch(hs_ctx *c, Batch &B, int n, const hs_batch_cfg *cfg, int reps, int workers,
                  double *roi_ms, double *kern_ms) {
   const int S = worker_waves(cfg);
-  const BatchFn kfn = pick_batch(cfg->mode, S);
+  const BatchFn kfn = pick_batch(cfg);
   if (!kfn) return HS_EINVAL;
   BArgs a;
   a.inst = B.d_inst;
@@ -409,7 +511,7 @@ int default_workers(const hs_ctx *c, const hs_batch_cfg *cfg, int nv) {
 int check_workers(hs_ctx *c, const hs_batch_cfg *cfg, int P, int nv) {
   const int S = worker_waves(cfg);
   int fit = 0, rc;
-  if ((rc = coresident_wgs(c, (const void *)pick_batch(cfg->mode, S), 64 * S, &fit))) return rc;
+  if ((rc = coresident_wgs(c, (const void *)pick_batch(cfg), 64 * S, &fit))) return rc;
   return fit < std::min(P, nv) ? HS_EOCCUPANCY : HS_OK;
 }
 
@@ -420,6 +522,7 @@ extern "C" int hs_batch_run(hs_ctx *c, const hs_batch_instance *in, int n,
                             hs_batch_result *res) {
   if (!c || !in || n < 1 || !cfg || !storages || !entries || !res) return HS_EINVAL;
   if (worker_waves(cfg) == 0) return HS_EINVAL;  // workers of 64, 128 or 256 lanes
+  if (cfg->probe != 0 && cfg->probe != 1) return HS_EINVAL;
   if (cfg->mode != 0 && cfg->mode != 1) return HS_EINVAL;
   std::memset(res, 0, sizeof *res);
   const int reps = cfg->reps > 0 ? cfg->reps : 10;
@@ -467,6 +570,7 @@ extern "C" int hs_batch_autotune(hs_ctx *c, const hs_batch_instance *in, int n,
   hs_batch_cfg best_cfg = *cfg;
   best_cfg.wg_size = 64;
   best_cfg.backoff_min = 1;
+  best_cfg.probe = 0;
   int best_wpc = 8;
   // One calibration launch of t; *ms < 0 when its workers cannot all be resident.
   auto trial = [&](const hs_batch_cfg &t, double *ms) -> int {
@@ -509,14 +613,16 @@ extern "C" int hs_batch_autotune(hs_ctx *c, const hs_batch_instance *in, int n,
         }
       }
   // stage 2, at the same waves per CU: worker width (lanes sharing each
-  // critical section) x backoff start I_min
+  // critical section) x backoff start I_min x attempt form (serial, probed)
   static const int kWidth[] = {64, 128, 256};
   static const int kBackoffMin[] = {1, 4, 16};
   const hs_batch_cfg base = best_cfg;
+  for (int probe : {0, 1})
   for (int wd : kWidth)
     for (int bmin : kBackoffMin) {
-      if (wd == 64 && bmin == 1) continue;  // stage 1's winner
+      if (!probe && wd == 64 && bmin == 1) continue;  // stage 1's winner
       hs_batch_cfg t = base;
+      t.probe = probe;
       t.wg_size = wd;
       t.persistent_wgs = std::max(1, best_wpc * c->num_cu / (wd / 64));
       t.backoff_min = bmin;

@@ -264,6 +264,10 @@ typedef struct {
   int persistent_wgs; /* worker WGs of the persistent grid; 0 = 8 waves
                          per CU */
   double timeout_s;
+  int probe;         /* attempt form: 0 = lane 0 tries each virtual WG in
+                        turn (one round trip each), 1 = one load per lane
+                        probes all of a worker's virtual WGs at once, then
+                        lane 0 acts on the hopeful ones */
 } hs_batch_cfg;
 
 typedef struct {
@@ -284,9 +288,10 @@ int hs_batch_run(hs_ctx *ctx, const hs_batch_instance *inst, int n,
  * iterations capped at 20, one launch, in cfg->mode) over waves per CU
  * {2,4,8,16} x poll interval {0,1,4} x EBO cap I_max {63,255,1023} with
  * 64-lane workers, then, at the winning waves per CU, worker width
- * {64,128,256} x backoff start I_min {1,4,16}.  Writes the fastest
- * wg_size / persistent_wgs / poll_sleep / backoff_min / backoff_max into
- * *cfg (grids that cannot be co-resident are skipped). */
+ * {64,128,256} x backoff start I_min {1,4,16} x attempt form (probe 0,
+ * 1).  Writes the fastest wg_size / persistent_wgs / poll_sleep /
+ * backoff_min / backoff_max / probe into *cfg (grids that cannot be
+ * co-resident are skipped). */
 int hs_batch_autotune(hs_ctx *ctx, const hs_batch_instance *inst, int n,
                       hs_batch_cfg *cfg, double *best_ms);
 

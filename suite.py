@@ -57,7 +57,7 @@ def c2(dev, terms):
 
 
 def c3(dev, terms):
-    return bench.extra_c3(dev, hs)
+    return bench.extra_c3(dev, hs, terms)
 
 
 def c4(dev, terms, iters=10000):

@@ -60,6 +60,7 @@ def test_batch_full_mix_inc_autotuned(dev, ohelp):
     mix = make_mix(seed=3, n=64)
     t = dev.batch_autotune(mix, reps=1, mode=1)
     assert t["wg_size"] in (64, 128, 256) and t["backoff_min"] in (1, 4, 16)
+    assert t["probe"] in (0, 1)
     assert t["persistent_wgs"] > 0 and t["best_ms"] > 0
     for _ in range(2):
         st = storages_for(dev, mix)
@@ -69,27 +70,32 @@ def test_batch_full_mix_inc_autotuned(dev, ohelp):
         check_mix(dev, ohelp, mix, st, entries, 1, 1)
 
 
+@pytest.mark.parametrize("probe", [0, 1])
 @pytest.mark.parametrize("mode", [0, 1])
-@pytest.mark.parametrize("wg_size", [128, 256])
-def test_batch_worker_width(dev, ohelp, wg_size, mode):
+@pytest.mark.parametrize("wg_size", [64, 128, 256])
+def test_batch_worker_width(dev, ohelp, wg_size, mode, probe):
     """Workers of 2 and 4 waves split each critical section's lines over
-    their waves (numLdSt 1, 10 and 100 all occur in the mix)."""
+    their waves (numLdSt 1, 10 and 100 all occur in the mix); both attempt
+    forms (lane 0 in turn, or one probe per lane)."""
     mix = make_mix(seed=23, n=48, iters_cap=30)
     assert {L for _, _, _, L in mix} == {1, 10, 100}
     st = storages_for(dev, mix)
-    entries, r = dev.batch(mix, st, reps=2, mode=mode, wg_size=wg_size, use_graph=1)
+    entries, r = dev.batch(mix, st, reps=2, mode=mode, wg_size=wg_size, use_graph=1,
+                           probe=probe)
     assert r.all_ok == 1
     check_mix(dev, ohelp, mix, st, entries, 2, mode)
 
 
+@pytest.mark.parametrize("probe", [0, 1])
 @pytest.mark.parametrize("seed", [17, 29, 41])
 @pytest.mark.parametrize("workers", [256, 2048])
-def test_batch_mix_seeds_and_workers(dev, ohelp, seed, workers):
+def test_batch_mix_seeds_and_workers(dev, ohelp, seed, workers, probe):
     """Other mixes, with the virtual WGs spread over few workers (many per
     worker) or many (a few each)."""
     mix = make_mix(seed=seed, n=32, iters_cap=15)
     st = storages_for(dev, mix)
-    entries, r = dev.batch(mix, st, reps=2, mode=1, use_graph=1, persistent_wgs=workers)
+    entries, r = dev.batch(mix, st, reps=2, mode=1, use_graph=1, persistent_wgs=workers,
+                           probe=probe)
     assert r.all_ok == 1
     check_mix(dev, ohelp, mix, st, entries, 2, 1)
 
@@ -106,7 +112,7 @@ def test_batch_alone_equals_plain(dev):
 
 def test_batch_rejects_bad_width_and_mode(dev):
     mix = make_mix(seed=11, n=2, iters_cap=2)
-    for kw in (dict(wg_size=96), dict(wg_size=512), dict(mode=2)):
+    for kw in (dict(wg_size=96), dict(wg_size=512), dict(mode=2), dict(probe=2)):
         with pytest.raises(hs.HSError) as ei:
             dev.batch(mix, storages_for(dev, mix), reps=1, **kw)
         assert ei.value.code == -1  # HS_EINVAL
