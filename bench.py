@@ -66,7 +66,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--only", default="",
-                    help="comma list of extras to run (barriers,c1,c3,c5); default all")
+                    help="comma list of extras to run (barriers,c1,c3,c5,uts_cpu); default all")
     ap.add_argument("--no-probe", action="store_true",
                     help="skip the floor probe (a child process); the roofline then has no "
                          "floor (for runs under a profiler)")
@@ -346,6 +346,31 @@ def extra_c5(dev, slots=16, load=0.8, n=256, seed=9, depth=(6, 14)):
             "saturation": sat, "poisson": pois, "parity_ok": bool(ok)}
 
 
+def extra_uts_host_workers(dev, seed=7, depth=13, threads=(0, 4, 16)):
+    """One large geometric tree (b0 = 4) searched by the GPU alone and with
+    host worker threads taking part (each served by one GPU WG through a
+    pinned mailbox): what the host's cores add to the GPU's rate."""
+    dev.uts_geo(1, 4.0, 4)
+    base = dev.uts_geo(seed, 4.0, depth)
+    out = {"workload": f"geometric tree seed {seed}, b0=4, depth {depth}: {base.nodes} nodes",
+           "runs": {}}
+    ok = True
+    for t in threads:
+        if t == 0:
+            r, cpu = dev.uts_geo(seed, 4.0, depth), None
+        else:
+            r, cpu = dev.uts_geo_cpu(seed, 4.0, depth, t)
+        ok = ok and (r.nodes, r.leaves, r.max_depth) == (base.nodes, base.leaves, base.max_depth)
+        line = {"nodes_per_s": round(r.nodes / (r.kernel_ms / 1e3), 1),
+                "kernel_ms": round(r.kernel_ms, 3)}
+        if cpu is not None:
+            line.update(host_nodes=int(cpu.nodes), host_share=round(cpu.nodes / r.nodes, 8),
+                        host_batches=int(cpu.deliveries), host_busy_s=round(cpu.busy_s, 4))
+        out["runs"][f"host_threads_{t}"] = line
+    out["parity_ok"] = bool(ok)
+    return out
+
+
 # ---- CPU baselines ------------------------------------------------------------
 
 def host_info():
@@ -469,7 +494,7 @@ def main():
         sys.exit(2)
     import heterosync_amd as hs
 
-    only = set(filter(None, a.only.split(","))) or {"barriers", "c1", "c3", "c5"}
+    only = set(filter(None, a.only.split(","))) or {"barriers", "c1", "c3", "c5", "uts_cpu"}
     # the probe runs first, in its own process, before this one opens the device
     ncu_guess = 256
     floor_terms = None if a.no_probe else probe_floor(4 * ncu_guess, a.wg, a.num_ldst)
@@ -553,6 +578,8 @@ def main():
             extras["c3_batched"] = extra_c3(dev, hs, floor_terms)
         if "c5" in only:
             extras["c5_serving"] = extra_c5(dev)
+        if "uts_cpu" in only:
+            extras["uts_host_workers"] = extra_uts_host_workers(dev)
         out["extras"] = extras
     dev.close()
     if not a.no_cpu:

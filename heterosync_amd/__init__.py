@@ -30,7 +30,8 @@ EXPORTED = [
     "hs_prim_name", "hs_prim_supported", "hs_storage_words", "hs_syncprim_expected",
     "hs_syncprim_run", "hs_syncprim_run_noalloc",
     "hs_seqlock_run", "hs_flags_run", "hs_refcounter_run", "hs_splitcounter_run",
-    "hs_uts_ref_run", "hs_uts_geo_run", "hs_batch_run", "hs_batch_autotune",
+    "hs_uts_ref_run", "hs_uts_geo_run", "hs_uts_ref_run_cpu", "hs_uts_geo_run_cpu",
+    "hs_batch_run", "hs_batch_autotune",
     "hs_uts_serve",
 ]
 
@@ -69,6 +70,11 @@ class UtsResult(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("nodes", ctypes.c_uint64),
                 ("leaves", ctypes.c_uint64), ("leaf_sum", ctypes.c_uint64),
                 ("max_depth", ctypes.c_int), ("aborted", ctypes.c_int)]
+
+
+class UtsCpuResult(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_uint64), ("deliveries", ctypes.c_uint64),
+                ("busy_s", ctypes.c_double), ("threads", ctypes.c_int)]
 
 
 class BatchInstance(ctypes.Structure):
@@ -159,6 +165,12 @@ def _lib():
                                        P(UtsResult)]
         lib.hs_uts_geo_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int, P(UtsResult)]
+        lib.hs_uts_ref_run_cpu.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_uint32),
+                                           P(ctypes.c_int32), P(ctypes.c_int32), ctypes.c_int,
+                                           ctypes.c_int, P(UtsResult), P(UtsCpuResult)]
+        lib.hs_uts_geo_run_cpu.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           P(UtsResult), P(UtsCpuResult)]
         lib.hs_batch_run.argtypes = [ctypes.c_void_p, P(BatchInstance), ctypes.c_int,
                                      P(BatchCfg), P(P(ctypes.c_int32)), P(ctypes.c_int64),
                                      P(BatchResult)]
@@ -326,6 +338,26 @@ class Device:
         _check(_lib().hs_uts_geo_run(self._h, seed, b0, gen_mx, num_blocks,
                                      ctypes.byref(res)), "hs_uts_geo_run")
         return res
+
+    def uts_geo_cpu(self, seed, b0, gen_mx, cpu_threads, num_blocks=0):
+        """A geometric tree with host worker threads taking part; returns
+        (UtsResult, UtsCpuResult)."""
+        res, cpu = UtsResult(), UtsCpuResult()
+        _check(_lib().hs_uts_geo_run_cpu(self._h, seed, b0, gen_mx, num_blocks, cpu_threads,
+                                         ctypes.byref(res), ctypes.byref(cpu)),
+               "hs_uts_geo_run_cpu")
+        return res, cpu
+
+    def uts_ref_cpu(self, value, child_a, child_b, cpu_threads, num_blocks=0):
+        res, cpu = UtsResult(), UtsCpuResult()
+        v = np.ascontiguousarray(value, np.uint32)
+        a = np.ascontiguousarray(child_a, np.int32)
+        b = np.ascontiguousarray(child_b, np.int32)
+        _check(_lib().hs_uts_ref_run_cpu(self._h, len(v), _ptr(v, ctypes.c_uint32),
+                                         _ptr(a, ctypes.c_int32), _ptr(b, ctypes.c_int32),
+                                         num_blocks, cpu_threads, ctypes.byref(res),
+                                         ctypes.byref(cpu)), "hs_uts_ref_run_cpu")
+        return res, cpu
 
     def uts_serve(self, requests, slots=0, num_blocks=0, deque_nodes=0, timeout_s=60.0):
         n = len(requests)

@@ -3,11 +3,15 @@
 // Builds the same binary tree on the host (glibc srand(200), uts.cu:290-339),
 // runs the GPU work-stealing search and verifies leaves, leaf sum and node
 // count against the host golden (uts.cu:412-438); prints SUCCESS! or exits
-// -1.  <numThreads> and <local_stack_size> are accepted for compatibility
-// (the GPU engine sizes its own per-workgroup deques); N_active_blocks is
-// the number of workgroups (0 = default).
+// -1.  <numThreads> host threads take part, as the reference's pthreads
+// were meant to (README.txt:30; its threads only meet at a barrier,
+// uts.cu:457-539): this thread launches the GPU search and numThreads - 1
+// host workers expand records the GPU hands them (hs_uts_ref_run_cpu).
+// <local_stack_size> is accepted for compatibility (the GPU engine sizes its
+// own per-workgroup deques); N_active_blocks is the number of workgroups
+// (0 = default).
 //
-// Geometric SHA-1 trees:  ./uts --geo <seed> <b0> <depth> [numBlocks]
+// Geometric SHA-1 trees:  ./uts --geo <seed> <b0> <depth> [numBlocks] [numThreads]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -70,10 +74,17 @@ int main(int argc, char **argv) {
     const double b0 = std::atof(argv[3]);
     const int depth = std::atoi(argv[4]);
     const int blocks = argc > 5 ? std::atoi(argv[5]) : 0;
+    const int threads = argc > 6 ? std::atoi(argv[6]) : 1;
+    if (threads < 1) { std::fprintf(stderr, "numThreads must be >= 1\n"); return -1; }
     if (hs_open(0, &ctx)) { std::fprintf(stderr, "no HIP device\n"); return -1; }
     hs_uts_result r;
-    const int rc = hs_uts_geo_run(ctx, seed, b0, depth, blocks, &r);
+    hs_uts_cpu_result cpu;
+    const int rc = hs_uts_geo_run_cpu(ctx, seed, b0, depth, blocks, threads - 1, &r, &cpu);
     if (rc) { std::fprintf(stderr, "ERROR hs_uts_geo_run - %s\n", hs_strerror(rc)); return -1; }
+    if (threads > 1)
+      std::printf("host workers: %d, nodes expanded on the host: %llu (%llu batches)\n",
+                  threads - 1, (unsigned long long)cpu.nodes,
+                  (unsigned long long)cpu.deliveries);
     std::printf("Tree size = %llu, tree depth = %d, num leaves = %llu (%.2f%%)\n",
                 (unsigned long long)r.nodes, r.max_depth, (unsigned long long)r.leaves,
                 r.nodes ? 100.0 * r.leaves / r.nodes : 0.0);
@@ -85,11 +96,13 @@ int main(int argc, char **argv) {
   if (argc != 6) {
     std::fprintf(stderr, "Invalid arguments passed in, expected:\n");
     std::fprintf(stderr, "./uts <numThreads> <numElements> <fertility> <local_stack_size> <N_active_blocks>\n");
-    std::fprintf(stderr, "   or ./uts --geo <seed> <b0> <depth> [numBlocks]\n");
+    std::fprintf(stderr, "   or ./uts --geo <seed> <b0> <depth> [numBlocks] [numThreads]\n");
     return -1;
   }
+  const int threads = std::atoi(argv[1]);
   const int N = std::atoi(argv[2]), fert = std::atoi(argv[3]), blocks = std::atoi(argv[5]);
   if (N <= 0) { std::fprintf(stderr, "N must be > 0!\n"); return -1; }
+  if (threads < 1) { std::fprintf(stderr, "numThreads must be >= 1\n"); return -1; }
   const int num_nodes = N + ((N % 2) ? 0 : 1);  // odd (uts.cu:577)
   std::printf("Running with N_elements=%d N_active_blocks=%d\n", num_nodes, blocks);
   std::vector<uint32_t> value;
@@ -99,11 +112,15 @@ int main(int argc, char **argv) {
   build_tree(num_nodes, fert, value, ca, cb, leaves, leaf_sum, restarts);
   if (hs_open(0, &ctx)) { std::fprintf(stderr, "no HIP device\n"); return -1; }
   hs_uts_result r;
+  hs_uts_cpu_result cpu;
   std::printf("Launch uts kernel...\n");
-  const int rc = hs_uts_ref_run(ctx, num_nodes, value.data(), ca.data(), cb.data(),
-                                blocks > 0 ? blocks : 0, &r);
+  const int rc = hs_uts_ref_run_cpu(ctx, num_nodes, value.data(), ca.data(), cb.data(),
+                                    blocks > 0 ? blocks : 0, threads - 1, &r, &cpu);
   if (rc) { std::fprintf(stderr, "ERROR hs_uts_ref_run - %s\n", hs_strerror(rc)); return -1; }
   std::printf("Kernel completed.\n");
+  if (threads > 1)
+    std::printf("host workers: %d, nodes expanded on the host: %llu (%llu batches)\n",
+                threads - 1, (unsigned long long)cpu.nodes, (unsigned long long)cpu.deliveries);
   std::printf("Created uts tree size %d with %u leaves, %u leaf sum, %d fertility, and %d restarts:\n",
               num_nodes, leaves, leaf_sum, fert, restarts);
   std::printf("kernel found %llu leaf nodes\n", (unsigned long long)r.leaves);

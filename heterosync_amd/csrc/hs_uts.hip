@@ -40,6 +40,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -197,6 +200,11 @@ struct UtsArgs {
   // submitted count (dev_ctl[0]) and the stop word (dev_ctl[32])
   uint32_t *dev_inbox;
   uint32_t *dev_ctl;
+  // host worker threads (single-tree runs, hs_uts_*_run_cpu): client c's
+  // mailbox is cpu_mail + c * kMailWords and its box cpu_box + c *
+  // cpu_box_cap * W (pinned, host-coherent); WG c * cpu_stride serves it
+  uint32_t *cpu_mail, *cpu_box;
+  int ncpu, cpu_stride, cpu_box_cap;
   // optional per-WG statistics (null: off), kStatWords each:
   // busy ticks, idle ticks, batches, nodes, donations, donated, steals,
   // stolen, pop ticks, generate ticks, donate ticks, generate rounds
@@ -204,6 +212,18 @@ struct UtsArgs {
 };
 
 constexpr int kStatWords = 12;
+
+// A host worker thread's mailbox (pinned host memory, one per thread).  The
+// host sets kMailWant when idle; its serving WG, seeing kMailWant with an
+// empty box, writes k records into the box and then k into kMailBox; the
+// host expands them depth-first, publishes its cumulative counts (nodes,
+// leaves, leaf sum, max depth) and returned credits, bumps kMailSeq, and
+// clears kMailBox.  The serving WG adds each new total's increment to the
+// tree's device counters and subtracts the returned credits.  Each word has
+// one writer per transition, so no atomic RMW crosses the host link.
+constexpr int kMailWords = 64;
+constexpr int kMailWant = 0, kMailBox = 16, kMailSeq = 32;
+constexpr int kMailNodes = 33, kMailLeaves = 35, kMailLsum = 37, kMailCred = 39, kMailMaxd = 40;
 
 // Most children one record can have: the geometric count is capped by the
 // threshold table (at most 100, the UTS MAXNUMCHILDREN); the reference tree
@@ -310,7 +330,7 @@ __device__ void dispatch(const UtsArgs &a, Guard &g) {
 // The work-stealing worker (one wave per WG); the kernels below wrap it.
 // STATS: per-WG timing and hand-off counters (a separate instantiation, so
 // the kernel that runs by default spends no registers on them).
-template <int TREE, bool SERVE, bool STATS>
+template <int TREE, bool SERVE, bool STATS, bool CPU = false>
 __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   constexpr int W = Rec<TREE>::W;
   __shared__ uint32_t s_pst[W * kWave];   // parent records of the batch
@@ -536,6 +556,84 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
     if (lane == 0) st_rel(a.wbox_n + (long)v * HS_LINE_WORDS, (uint32_t)k);
   };
 
+  // ---- host workers (CPU): the serving WG's side of the mailbox ------------
+  const int cpu_client = (CPU && me % a.cpu_stride == 0) ? me / a.cpu_stride : -1;
+  const bool serves_cpu = CPU && cpu_client >= 0 && cpu_client < a.ncpu;
+  uint32_t *mail = serves_cpu ? a.cpu_mail + (long)cpu_client * kMailWords : nullptr;
+  unsigned long long ap_nodes = 0, ap_leaves = 0, ap_lsum = 0;  // totals applied so far
+  uint32_t ap_cred = 0, ap_seq = 0;
+  constexpr int kSys = __HIP_MEMORY_SCOPE_SYSTEM;
+  auto sys_ld = [&](const uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, kSys); };
+  // Applies the host's new totals: counts first, then the credits (release),
+  // as settle() does.  Whole wave; lane 0 works.
+  auto cpu_collect = [&]() {
+    if (!serves_cpu) return;
+    if (lane == 0) {
+      const uint32_t seq = sys_ld(mail + kMailSeq);
+      if (seq != ap_seq) {
+        auto u64 = [&](int k) {
+          return (unsigned long long)__hip_atomic_load(mail + k, __ATOMIC_RELAXED, kSys) |
+                 ((unsigned long long)__hip_atomic_load(mail + k + 1, __ATOMIC_RELAXED, kSys)
+                  << 32);
+        };
+        const unsigned long long nn = u64(kMailNodes), ll = u64(kMailLeaves), ls = u64(kMailLsum);
+        const uint32_t cr = __hip_atomic_load(mail + kMailCred, __ATOMIC_RELAXED, kSys);
+        const uint32_t md = __hip_atomic_load(mail + kMailMaxd, __ATOMIC_RELAXED, kSys);
+        if (nn != ap_nodes) atomicAdd(a.nodes, nn - ap_nodes);
+        if (ll != ap_leaves) atomicAdd(a.leaves, ll - ap_leaves);
+        if (ls != ap_lsum) atomicAdd(a.leafsum, ls - ap_lsum);
+        if (md) atomicMax(a.maxd, md);
+        if (cr != ap_cred)
+          __hip_atomic_fetch_sub(a.credit, cr - ap_cred, __ATOMIC_ACQ_REL, HS_AGENT);
+        ap_nodes = nn;
+        ap_leaves = ll;
+        ap_lsum = ls;
+        ap_cred = cr;
+        ap_seq = seq;
+      }
+    }
+  };
+  // Hands the host worker up to 64 of the newest records (the deepest, so
+  // the smallest subtrees) when it asks and its box is empty; they leave
+  // with one credit each.  Whole wave.
+  auto cpu_deliver = [&]() {
+    if (!serves_cpu) return;
+    int k = 0;
+    if (lane == 0 && sys_ld(mail + kMailWant) != 0 && sys_ld(mail + kMailBox) == 0)
+      k = (int)std::min<long>(std::min<long>(top - bot - 2 * kWave, (long)kWave),
+                              (long)a.cpu_box_cap);
+    k = __builtin_amdgcn_readlane(k, 0);
+    if (k <= 0) return;
+    flush_top();  // the newest records go; the deque must hold them
+    uint32_t *box = a.cpu_box + (long)cpu_client * a.cpu_box_cap * W;
+    uint32_t sl = 0;
+    const bool mine = lane < k;
+    if (mine) {
+      const long at = (top - k + lane) & dq_mask;
+      uint32_t r[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) r[w] = deq[dq_idx<W>(at, w)];
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        __hip_atomic_store(box + dq_idx<W>(lane, w), r[w], __ATOMIC_RELAXED, kSys);
+      sl = meta_slot(r[W - 1]);
+    }
+    slot_count_add(s_hold, s_cred, sl, mine, -1);
+    top -= k;
+    __syncthreads();
+    for (int s2 = lane; s2 < kMaxSlots; s2 += kWave) {  // as donate(): keep >= 1 credit
+      const int want = (s_hold[s2] > 0 || s_nodes[s2] != 0) ? 1 : 0;
+      if (s_cred[s2] < want) {
+        __hip_atomic_fetch_add(a.credit + (long)s2 * HS_LINE_WORDS,
+                               (uint32_t)(want - s_cred[s2]), __ATOMIC_RELAXED, HS_AGENT);
+        s_cred[s2] = want;
+      }
+    }
+    __syncthreads();
+    // the records (this wave's stores), then the count: a system-scope release
+    if (lane == 0) __hip_atomic_store(mail + kMailBox, (uint32_t)k, __ATOMIC_RELEASE, kSys);
+  };
+
   for (;;) {
     const long held = top - bot;
     if (held > 0) {
@@ -720,6 +818,10 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
         if (k_in > 0) absorb(k_in);  // waits in the inbox while it does not fit
       }
       if ((batches & 1) == 0 && top - bot >= 2 * kWave) donate();
+      if (CPU && (batches & 7) == 0 && top - bot > 3 * kWave) {
+        cpu_collect();
+        cpu_deliver();
+      }
       // return the credits of trees this WG no longer holds, so that a tree
       // finishes while its last holders work on other trees
       if (SERVE && (batches & 7) == 0) settle();
@@ -734,6 +836,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
 
     // ---- no local work: settle, then look for more -------------------------
     settle();
+    cpu_collect();  // a host worker's returned credits may end the tree
     {
       const int k_in = (int)__builtin_amdgcn_readlane(lane == 0 ? ld_rlx(my_n) : 0u, 0);
       if (k_in > 0 && absorb(k_in)) continue;  // an empty deque takes any delivery
@@ -841,6 +944,13 @@ k_uts_stats(UtsArgs a) {
 // nothing from a bigger grid), so it keeps its full register allocation.
 __global__ void __launch_bounds__(64) k_uts_serve(UtsArgs a) { uts_worker<1, true, false>(a); }
 
+// One tree with host worker threads taking part (hs_uts_*_run_cpu).
+template <int TREE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+k_uts_cpu(UtsArgs a) {
+  uts_worker<TREE, false, false, true>(a);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -881,6 +991,7 @@ struct Engine {
   int tree = -1, blocks = 0, W = 0, dq_log2 = 0, cap = 0;
   double b0 = 0;
   uint32_t *d_thr = nullptr;
+  std::vector<uint32_t> h_thr;  // the same thresholds, for host workers
   uint32_t *state = nullptr;
   size_t state_words = 0;
   uint32_t *results = nullptr;  // kResultWords words inside `state`
@@ -944,11 +1055,10 @@ struct Engine {
     a.live = state + o_live;
     a.admitted = a.live + 32;
     if (tree == 1) {
-      std::vector<uint32_t> thr;
       a.inv_log2q = (float)(1.0 / std::log2(1.0 - 1.0 / (1.0 + b0)));
-      a.nthr = geo_thresholds(b0, thr);
+      a.nthr = geo_thresholds(b0, h_thr);
       if (a.nthr <= 0 || a.nthr > kMaxThr) return HS_EINVAL;
-      HS_HIP(hipMemcpy(d_thr, thr.data(), thr.size() * 4, hipMemcpyHostToDevice));
+      HS_HIP(hipMemcpy(d_thr, h_thr.data(), h_thr.size() * 4, hipMemcpyHostToDevice));
     }
     a.thr = d_thr;
     a.ticks = timeout_ticks(c, 0);
@@ -1072,27 +1182,212 @@ int stats_report(Engine &e, double kernel_ms) {
   return HS_OK;
 }
 
+// ---- host worker threads ----------------------------------------------------
+
+// SHA-1 (FIPS 180-4) of one padded 64-byte block on the host, with the word
+// layout of the device's sha1_block.
+inline uint32_t host_rotl(uint32_t x, int n) { return (x << n) | (x >> (32 - n)); }
+
+void host_sha1_block(uint32_t h[5], uint32_t w[16]) {
+  uint32_t a = 0x67452301u, b = 0xefcdab89u, c = 0x98badcfeu, d = 0x10325476u, e = 0xc3d2e1f0u;
+  for (int t = 0; t < 80; ++t) {
+    uint32_t wt;
+    if (t < 16) {
+      wt = w[t];
+    } else {
+      wt = host_rotl(w[(t - 3) & 15] ^ w[(t - 8) & 15] ^ w[(t - 14) & 15] ^ w[t & 15], 1);
+      w[t & 15] = wt;
+    }
+    uint32_t f, k;
+    if (t < 20) {
+      f = (b & c) | (~b & d);
+      k = 0x5a827999u;
+    } else if (t < 40) {
+      f = b ^ c ^ d;
+      k = 0x6ed9eba1u;
+    } else if (t < 60) {
+      f = (b & c) | (b & d) | (c & d);
+      k = 0x8f1bbcdcu;
+    } else {
+      f = b ^ c ^ d;
+      k = 0xca62c1d6u;
+    }
+    const uint32_t tmp = host_rotl(a, 5) + f + e + k + wt;
+    e = d;
+    d = c;
+    c = host_rotl(b, 30);
+    b = a;
+    a = tmp;
+  }
+  h[0] = 0x67452301u + a;
+  h[1] = 0xefcdab89u + b;
+  h[2] = 0x98badcfeu + c;
+  h[3] = 0x10325476u + d;
+  h[4] = 0xc3d2e1f0u + e;
+}
+
+// What a host worker needs to expand a record.
+struct HostTree {
+  int tree = 1;
+  const uint32_t *thr = nullptr;  // geometric: child-count thresholds
+  int nthr = 0;
+  uint32_t gen_mx = 0;
+  const uint32_t *value = nullptr;  // reference tree arrays
+  const int32_t *child_a = nullptr, *child_b = nullptr;
+};
+
+struct HostTotals {
+  uint64_t nodes = 0, leaves = 0, lsum = 0;
+  uint32_t maxd = 0;
+};
+
+// Depth-first search of one record's subtree on the host: the device's
+// expansion (the same SHA-1 children and threshold counts, or the
+// reference tree's arrays), counted the same way.
+void host_expand(const HostTree &t, const uint32_t *rec, HostTotals &tot,
+                 std::vector<uint32_t> &stack) {
+  const int W = t.tree == 1 ? 6 : 2;
+  stack.assign(rec, rec + W);
+  while (!stack.empty()) {
+    uint32_t r[6];
+    std::copy(stack.end() - W, stack.end(), r);
+    stack.resize(stack.size() - W);
+    ++tot.nodes;
+    if (t.tree == 1) {
+      const uint32_t meta = r[5], depth = meta & 0xffffu;
+      tot.maxd = std::max(tot.maxd, depth);
+      const uint32_t h = r[4] & 0x7fffffffu;
+      const int nc = (depth < t.gen_mx || depth == 0)
+                         ? (int)(std::upper_bound(t.thr, t.thr + t.nthr, h) - t.thr)
+                         : 0;
+      if (nc == 0) ++tot.leaves;
+      for (int i = 0; i < nc; ++i) {  // child i = SHA1(parent || i)
+        uint32_t w[16] = {r[0], r[1], r[2], r[3], r[4], (uint32_t)i, 0x80000000u, 0, 0, 0, 0, 0,
+                          0, 0, 0, 192u};
+        uint32_t ch[5];
+        host_sha1_block(ch, w);
+        stack.insert(stack.end(), ch, ch + 5);
+        stack.push_back(meta + 1);
+      }
+    } else {
+      const uint32_t node = r[0];
+      if (t.child_a[node] == 0) {
+        ++tot.leaves;
+        tot.lsum += t.value[node];
+      } else {
+        stack.push_back((uint32_t)t.child_a[node]);
+        stack.push_back(r[1]);
+        stack.push_back((uint32_t)t.child_b[node]);
+        stack.push_back(r[1]);
+      }
+    }
+  }
+}
+
+// One host worker: asks for records, expands them, publishes its totals.
+void host_worker(uint32_t *mail, const uint32_t *box, int W, const HostTree &t,
+                 const std::atomic<int> *stop, hs_uts_cpu_result *acc, std::mutex *mu) {
+  HostTotals tot;
+  uint32_t cred = 0, seq = 0;
+  uint64_t deliveries = 0;
+  double busy = 0;
+  std::vector<uint32_t> stack, rec(W);
+  auto put = [&](int k, uint32_t v, int order) { __atomic_store_n(mail + k, v, order); };
+  put(kMailWant, 1, __ATOMIC_RELEASE);
+  while (!stop->load(std::memory_order_acquire)) {
+    const uint32_t k = __atomic_load_n(mail + kMailBox, __ATOMIC_ACQUIRE);
+    if (k == 0) {
+      std::this_thread::yield();
+      continue;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 0; i < k; ++i) {
+      for (int w = 0; w < W; ++w) rec[w] = box[((i >> 6) * W + w) * 64 + (i & 63)];
+      host_expand(t, rec.data(), tot, stack);
+    }
+    cred += k;
+    put(kMailNodes, (uint32_t)tot.nodes, __ATOMIC_RELAXED);
+    put(kMailNodes + 1, (uint32_t)(tot.nodes >> 32), __ATOMIC_RELAXED);
+    put(kMailLeaves, (uint32_t)tot.leaves, __ATOMIC_RELAXED);
+    put(kMailLeaves + 1, (uint32_t)(tot.leaves >> 32), __ATOMIC_RELAXED);
+    put(kMailLsum, (uint32_t)tot.lsum, __ATOMIC_RELAXED);
+    put(kMailLsum + 1, (uint32_t)(tot.lsum >> 32), __ATOMIC_RELAXED);
+    put(kMailCred, cred, __ATOMIC_RELAXED);
+    put(kMailMaxd, tot.maxd, __ATOMIC_RELAXED);
+    put(kMailSeq, ++seq, __ATOMIC_RELEASE);  // the totals, then their sequence
+    put(kMailBox, 0, __ATOMIC_RELEASE);      // the box may be refilled
+    ++deliveries;
+    busy += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  std::lock_guard<std::mutex> lk(*mu);
+  acc->nodes += tot.nodes;
+  acc->deliveries += deliveries;
+  acc->busy_s += busy;
+}
+
 // One run: clear the state, deliver the root, run the kernel, read slot 0.
 // depth: how many levels the tree goes below its root (the deque bound).
+// cpu_threads > 0: that many host workers take part (k_uts_cpu); `host`
+// describes the tree to them.
 template <int TREE>
 int run_single(hs_ctx *c, Engine &e, uint32_t seed, uint32_t gen_mx, long depth,
-               hs_uts_result *res) {
+               hs_uts_result *res, int cpu_threads = 0, HostTree host = HostTree(),
+               hs_uts_cpu_result *cpu = nullptr) {
   int rc;
   if ((rc = e.bound(depth))) return rc;
   if ((rc = e.reset(c->stream))) return rc;
   if ((rc = stats_begin(e, c->stream))) return rc;
+  e.a.ncpu = 0;
+  e.a.cpu_mail = e.a.cpu_box = nullptr;
+  struct Pinned {  // host workers' mailboxes and boxes
+    void *p[2] = {nullptr, nullptr};
+    ~Pinned() { for (void *q : p) if (q) (void)hipHostFree(q); }
+  } pinned;
+  const int box_cap = kWave;
+  if (cpu_threads > 0) {
+    const unsigned kPinFlags = hipHostMallocCoherent | hipHostMallocMapped;
+    HS_HIP(hipHostMalloc(&pinned.p[0], (size_t)cpu_threads * kMailWords * 4, kPinFlags));
+    HS_HIP(hipHostMalloc(&pinned.p[1], (size_t)cpu_threads * box_cap * e.W * 4, kPinFlags));
+    std::memset(pinned.p[0], 0, (size_t)cpu_threads * kMailWords * 4);
+    e.a.cpu_mail = static_cast<uint32_t *>(pinned.p[0]);
+    e.a.cpu_box = static_cast<uint32_t *>(pinned.p[1]);
+    e.a.ncpu = cpu_threads;
+    e.a.cpu_stride = std::max(1, e.blocks / cpu_threads);
+    e.a.cpu_box_cap = box_cap;
+  }
   hipLaunchKernelGGL(k_seed_root, dim3(1), dim3(kWave), 0, c->stream, e.a, TREE, seed, gen_mx);
   HS_HIP(hipGetLastError());
   EventSet ev;
   if ((rc = ev.create(2))) return rc;
+  std::atomic<int> stop{0};
+  std::mutex mu;
+  hs_uts_cpu_result dummy;  // the workers' totals when the caller wants none
+  std::memset(&dummy, 0, sizeof dummy);
+  std::vector<std::thread> workers;  // joined below before any return
+  if (cpu_threads > 0) {
+    HS_HIP(hipStreamSynchronize(c->stream));
+    if (cpu) std::memset(cpu, 0, sizeof *cpu);
+    for (int t = 0; t < cpu_threads; ++t)
+      workers.emplace_back(host_worker, e.a.cpu_mail + (size_t)t * kMailWords,
+                           e.a.cpu_box + (size_t)t * box_cap * e.W, e.W, std::cref(host), &stop,
+                           cpu ? cpu : &dummy, &mu);
+  }
   HS_HIP(hipEventRecord(ev[0], c->stream));
-  if (e.a.stats)
+  if (cpu_threads > 0)
+    hipLaunchKernelGGL((k_uts_cpu<TREE>), dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
+  else if (e.a.stats)
     hipLaunchKernelGGL((k_uts_stats<TREE>), dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
   else
     hipLaunchKernelGGL((k_uts<TREE>), dim3(e.blocks), dim3(kWave), 0, c->stream, e.a);
-  HS_HIP(hipGetLastError());
-  HS_HIP(hipEventRecord(ev[1], c->stream));
-  HS_HIP(hipEventSynchronize(ev[1]));
+  const hipError_t le = hipGetLastError();
+  const hipError_t re = hipEventRecord(ev[1], c->stream);
+  const hipError_t se = hipEventSynchronize(ev[1]);
+  stop.store(1, std::memory_order_release);
+  for (auto &w : workers) w.join();
+  if (cpu) cpu->threads = cpu_threads;
+  HS_HIP(le);
+  HS_HIP(re);
+  HS_HIP(se);
   float ms = 0;
   HS_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
   res->kernel_ms = ms;
@@ -1104,10 +1399,13 @@ void free_engine(void *p) { delete static_cast<Engine *>(p); }
 
 }  // namespace
 
-extern "C" int hs_uts_ref_run(hs_ctx *c, int num_nodes, const uint32_t *value,
-                              const int32_t *child_a, const int32_t *child_b, int num_blocks,
-                              hs_uts_result *res) {
-  if (!c || !value || !child_a || !child_b || !res || num_nodes < 1 || num_blocks < 0)
+namespace {
+
+int uts_ref(hs_ctx *c, int num_nodes, const uint32_t *value, const int32_t *child_a,
+            const int32_t *child_b, int num_blocks, int cpu_threads, hs_uts_result *res,
+            hs_uts_cpu_result *cpu) {
+  if (!c || !value || !child_a || !child_b || !res || num_nodes < 1 || num_blocks < 0 ||
+      cpu_threads < 0)
     return HS_EINVAL;
   std::memset(res, 0, sizeof *res);
   for (int i = 0; i < num_nodes; ++i)  // binary tree, children in range
@@ -1146,15 +1444,22 @@ extern "C" int hs_uts_ref_run(hs_ctx *c, int num_nodes, const uint32_t *value,
   e.a.value = d_val;
   e.a.child_a = d_a;
   e.a.child_b = d_b;
-  return run_single<0>(c, e, 0, 0, height, res);
+  if (cpu_threads > e.blocks) return HS_EINVAL;  // one serving WG per host worker
+  HostTree host;
+  host.tree = 0;
+  host.value = value;
+  host.child_a = child_a;
+  host.child_b = child_b;
+  return run_single<0>(c, e, 0, 0, height, res, cpu_threads, host, cpu);
 }
 
 // The geometric engine of a context is kept between calls (same grid, deque
 // size and b0), so a call costs a state fill, the root delivery, the kernel
 // and one small copy back.
-extern "C" int hs_uts_geo_run(hs_ctx *c, int seed, double b0, int gen_mx, int num_blocks,
-                              hs_uts_result *res) {
-  if (!c || !res || !(b0 > 0 && b0 < 1000) || gen_mx < 0 || gen_mx > 60000 || num_blocks < 0)
+int uts_geo(hs_ctx *c, int seed, double b0, int gen_mx, int num_blocks, int cpu_threads,
+            hs_uts_result *res, hs_uts_cpu_result *cpu) {
+  if (!c || !res || !(b0 > 0 && b0 < 1000) || gen_mx < 0 || gen_mx > 60000 || num_blocks < 0 ||
+      cpu_threads < 0)
     return HS_EINVAL;
   std::memset(res, 0, sizeof *res);
   HS_HIP(hipSetDevice(c->device));
@@ -1173,7 +1478,38 @@ extern "C" int hs_uts_geo_run(hs_ctx *c, int seed, double b0, int gen_mx, int nu
     c->uts_cache = e;
     c->uts_cache_free = free_engine;
   }
-  return run_single<1>(c, *e, (uint32_t)seed, (uint32_t)gen_mx, gen_mx, res);
+  if (cpu_threads > e->blocks) return HS_EINVAL;  // one serving WG per host worker
+  HostTree host;
+  host.tree = 1;
+  host.thr = e->h_thr.data();
+  host.nthr = (int)e->h_thr.size();
+  host.gen_mx = (uint32_t)gen_mx;
+  return run_single<1>(c, *e, (uint32_t)seed, (uint32_t)gen_mx, gen_mx, res, cpu_threads, host,
+                       cpu);
+}
+
+}  // namespace
+
+extern "C" int hs_uts_ref_run(hs_ctx *c, int num_nodes, const uint32_t *value,
+                              const int32_t *child_a, const int32_t *child_b, int num_blocks,
+                              hs_uts_result *res) {
+  return uts_ref(c, num_nodes, value, child_a, child_b, num_blocks, 0, res, nullptr);
+}
+
+extern "C" int hs_uts_ref_run_cpu(hs_ctx *c, int num_nodes, const uint32_t *value,
+                                  const int32_t *child_a, const int32_t *child_b, int num_blocks,
+                                  int cpu_threads, hs_uts_result *res, hs_uts_cpu_result *cpu) {
+  return uts_ref(c, num_nodes, value, child_a, child_b, num_blocks, cpu_threads, res, cpu);
+}
+
+extern "C" int hs_uts_geo_run(hs_ctx *c, int seed, double b0, int gen_mx, int num_blocks,
+                              hs_uts_result *res) {
+  return uts_geo(c, seed, b0, gen_mx, num_blocks, 0, res, nullptr);
+}
+
+extern "C" int hs_uts_geo_run_cpu(hs_ctx *c, int seed, double b0, int gen_mx, int num_blocks,
+                                  int cpu_threads, hs_uts_result *res, hs_uts_cpu_result *cpu) {
+  return uts_geo(c, seed, b0, gen_mx, num_blocks, cpu_threads, res, cpu);
 }
 
 extern "C" int hs_uts_serve(hs_ctx *c, const hs_uts_request *req, int n, const hs_serve_cfg *cfg,

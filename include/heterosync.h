@@ -23,6 +23,8 @@
  *   hs_refcounter_run          cuda/relaxedAtomics/uvm/refCounter/refCounter.cu:5-155
  *   hs_splitcounter_run        cuda/relaxedAtomics/uvm/splitCounter/splitCounter.cu:5-93
  *   hs_uts_ref_run             cuda/uts/uvm/uts.cu:244-455 (callGPU/doWork)
+ *   hs_uts_*_run_cpu           the same with host worker threads taking part
+ *                              (uts.cu:457-539's pthreads, README.txt:30)
  *   hs_uts_geo_run             no reference implementation (SHA-1 UTS,
  *                              the public UTS algorithm; see hs_uts_geo_run)
  *   hs_batch_run, hs_uts_serve extensions with no reference counterpart:
@@ -236,6 +238,32 @@ int hs_uts_ref_run(hs_ctx *ctx, int num_nodes, const uint32_t *value,
  * have at most 96 children). */
 int hs_uts_geo_run(hs_ctx *ctx, int seed, double b0, int gen_mx,
                    int num_blocks, hs_uts_result *res);
+
+/* Host worker threads taking part in one search: README.txt:30 promises
+ * load balancing "across CPU and GPU threads"; the reference's pthreads only
+ * meet at a barrier (cuda/uts/uvm/uts.cu:457-539).  Here each of
+ * cpu_threads host threads has a mailbox in pinned, host-coherent memory
+ * served by one GPU workgroup: when the thread asks, its WG hands it up to
+ * 64 of its newest records (with their termination credits); the thread
+ * expands their subtrees depth-first (the same SHA-1 children, threshold
+ * counts or reference-tree arrays), then publishes its cumulative counts
+ * and returned credits, which its WG folds into the tree's device counters.
+ * The counts are the GPU-only run's exactly.  cpu_threads <= the grid's WGs.
+ * *cpu (may be NULL) receives what the host workers did. */
+typedef struct {
+  uint64_t nodes;       /* nodes the host workers expanded */
+  uint64_t deliveries;  /* record batches they received */
+  double busy_s;        /* their summed expansion time */
+  int threads;
+} hs_uts_cpu_result;
+
+int hs_uts_ref_run_cpu(hs_ctx *ctx, int num_nodes, const uint32_t *value,
+                       const int32_t *child_a, const int32_t *child_b,
+                       int num_blocks, int cpu_threads, hs_uts_result *res,
+                       hs_uts_cpu_result *cpu);
+int hs_uts_geo_run_cpu(hs_ctx *ctx, int seed, double b0, int gen_mx,
+                       int num_blocks, int cpu_threads, hs_uts_result *res,
+                       hs_uts_cpu_result *cpu);
 
 /* ---- extensions ---------------------------------------------------------- */
 

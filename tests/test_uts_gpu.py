@@ -168,3 +168,34 @@ def test_uts_geo_depth_past_the_deque_bound_is_refused(dev, oracle):
     for seed in (3, 1612, 2003):
         r = dev.uts_geo(seed, b0, d)
         assert (r.nodes, r.leaves, r.max_depth) == geo_count(oracle, seed, b0, d)
+
+
+@pytest.mark.parametrize("seed,d,threads", [(19, 10, 4), (42, 10, 16), (7, 12, 8), (3, 13, 16)])
+def test_uts_geo_with_host_workers(dev, oracle, seed, d, threads):
+    """Host worker threads take part in the search (each served by one GPU
+    WG through a pinned mailbox): the counts are the GPU-only run's and the
+    oracle's, and the host workers expanded part of the tree."""
+    r, cpu = dev.uts_geo_cpu(seed, 4.0, d, threads)
+    alone = dev.uts_geo(seed, 4.0, d)
+    assert (r.nodes, r.leaves, r.max_depth) == (alone.nodes, alone.leaves, alone.max_depth)
+    if d <= 10:
+        assert (r.nodes, r.leaves, r.max_depth) == geo_count(oracle, seed, 4.0, d)
+    assert cpu.threads == threads
+    assert cpu.nodes < r.nodes
+    if alone.nodes > 10_000_000:  # a run long enough for the host to be handed work
+        assert cpu.deliveries > 0 and cpu.nodes > 0
+
+
+@pytest.mark.parametrize("n,fert,threads", [(200001, 60, 4), (1000001, 95, 16)])
+def test_uts_reference_tree_with_host_workers(dev, oracle, n, fert, threads):
+    val, ca, cb, lg, ls = ref_tree(oracle, n, fert)
+    r, cpu = dev.uts_ref_cpu(val, ca, cb, threads)
+    assert (r.nodes, r.leaves, r.leaf_sum) == (n, lg, ls)
+    assert cpu.nodes < n
+
+
+def test_uts_host_workers_refused_beyond_the_grid(dev):
+    import heterosync_amd as hs
+    with pytest.raises(hs.HSError) as ei:
+        dev.uts_geo_cpu(1, 4.0, 4, 200, num_blocks=64)
+    assert ei.value.code == -1
