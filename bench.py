@@ -169,13 +169,15 @@ def extra_c1(dev, hs, floor_terms):
     s, r = dev.syncprim("faMutex", 10, 32, 100, reps=10, mode=1, timeout_s=30.0)
     want = hs.expected_storage("faMutex", 10, 32, 100, 10, 64, dev.info.num_cu, 1)
     per_us = r.roi_ms * 1e3 / r.cs_entries
-    fl = handoff_floor_us(floor_terms, 10)
-    return {"workload": "faMutex 32 WGs x 64, numLdSt=10, numCSIters=100, 10 launches, inc mode",
-            "acq_rel_per_s": round(r.cs_entries / (r.roi_ms / 1e3), 1),
-            "us_per_acquire": round(per_us, 4), "floor_us": round(fl, 4),
-            "frac_of_floor": round(fl / per_us, 4),
-            "parity_ok": bool(np.array_equal(s, want)) and r.state_ok == 1
-            and r.cs_entries == 32000}
+    out = {"workload": "faMutex 32 WGs x 64, numLdSt=10, numCSIters=100, 10 launches, inc mode",
+           "acq_rel_per_s": round(r.cs_entries / (r.roi_ms / 1e3), 1),
+           "us_per_acquire": round(per_us, 4),
+           "parity_ok": bool(np.array_equal(s, want)) and r.state_ok == 1
+           and r.cs_entries == 32000}
+    if floor_terms:
+        fl = handoff_floor_us(floor_terms, 10)
+        out.update(floor_us=round(fl, 4), frac_of_floor=round(fl / per_us, 4))
+    return out
 
 
 def handoff_us(dev, L):

@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(1024) k_mutex(SPArgs a) {
     if (is_master()) s_ok = mutex_lock<KIND, AGENT>(lk, ring, a.ring_stride, ticket, a.bo, g);
     __syncthreads();
     if (!s_ok) return;
-    wg_move<MODE, AGENT, 32>(a.storage, tid0, a.L);
+    wg_move<MODE, AGENT, 32, true>(a.storage, tid0, a.L);
     if (AGENT) wait_mem();
     __syncthreads();
     if (is_master()) {
@@ -104,12 +104,12 @@ __global__ void __launch_bounds__(1024) k_sem(SPArgs a) {
       if (UNIQ) {  // main.hip.cpp:1040-1058: own region, then +numCU WGs
         int cur = b;
         for (int j = 0; j < per; ++j) {
-          wg_move<MODE, AGENT, 32>(a.storage, (long)cur * blockDim.x, a.L);
+          wg_move<MODE, AGENT, 32, true>(a.storage, (long)cur * blockDim.x, a.L);
           cur = (cur + nab) % grid;
         }
       } else {  // main.hip.cpp:903-915: regions 0..perCU-1
         for (int j = 0; j < per; ++j)
-          wg_move<MODE, AGENT, 32>(a.storage, (long)j * blockDim.x, a.L);
+          wg_move<MODE, AGENT, 32, true>(a.storage, (long)j * blockDim.x, a.L);
       }
     } else {
       wg_read<AGENT>(a.storage, (long)(UNIQ ? b : pw) * blockDim.x, a.L, sink);

@@ -593,18 +593,29 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
       }
     }
   };
-  // Hands the host worker up to 64 of the newest records (the deepest, so
-  // the smallest subtrees) when it asks and its box is empty; they leave
-  // with one credit each.  Whole wave.
+  // Hands the host worker up to 64 of the newest records when it asks and
+  // its box is empty; they leave with one credit each.  A host core expands
+  // about a million nodes a second, and the tree cannot finish while it
+  // holds records, so it gets only small subtrees: geometric records one
+  // level above the depth limit (their children are leaves), and at most 16
+  // reference-tree records (newest, so deepest).  Whole wave.
   auto cpu_deliver = [&]() {
     if (!serves_cpu) return;
     int k = 0;
     if (lane == 0 && sys_ld(mail + kMailWant) != 0 && sys_ld(mail + kMailBox) == 0)
-      k = (int)std::min<long>(std::min<long>(top - bot - 2 * kWave, (long)kWave),
+      k = (int)std::min<long>(std::min<long>(top - bot - 2 * kWave, TREE == 1 ? kWave : 16),
                               (long)a.cpu_box_cap);
     k = __builtin_amdgcn_readlane(k, 0);
     if (k <= 0) return;
     flush_top();  // the newest records go; the deque must hold them
+    if (TREE == 1) {  // the newest run of records at depth >= limit - 1
+      const uint32_t lim = s_gmx[0];
+      const bool ok_rec =
+          lane < k && meta_depth(deq[dq_idx<W>((top - 1 - lane) & dq_mask, W - 1)]) + 1 >= lim;
+      const unsigned long long bad = __ballot(lane < k && !ok_rec);
+      if (bad) k = (int)__builtin_ctzll(bad);
+      if (k <= 0) return;
+    }
     uint32_t *box = a.cpu_box + (long)cpu_client * a.cpu_box_cap * W;
     uint32_t sl = 0;
     const bool mine = lane < k;

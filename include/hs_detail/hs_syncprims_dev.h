@@ -221,9 +221,12 @@ __device__ __forceinline__ void wg_move(int32_t *s, long tid0, int L, bool keep_
 // atomicTreeBarrUniq, one box): numLdSt 10, 3.90 us per episode without the
 // barrier and 3.41 with it (copy mode 3.34); numLdSt 100, 34.0 without and
 // 32.8 with it (copy 32.6-32.8); numLdSt 1 and 4, 2.64 and 2.75 without it,
-// 2.72 and 2.84 with it.  The choice is per launch and compile-time inside
-// the walk: a run-time test between the loads and the stores measured
-// slower than either form.
+// 2.72 and 2.84 with it (on another box all within 1-2%).  The choice is per
+// launch and compile-time inside the walk: a run-time test between the
+// loads and the stores measured slower than either form.  The mutex and
+// semaphore walks (one band of 32 lines, the reference's 64-thread WGs)
+// always drain first: without it an inc-mode faMutex hand-off took 2.05 us
+// against 1.57 us in copy mode (32 WGs x 64, numLdSt 10); with it, 1.57.
 __host__ __device__ inline bool inc_band_barrier(int wg, int L, int band = 16) {
   if (wg <= 64 || L <= 0) return false;
   return L > band || band % L != 0;

@@ -13,7 +13,9 @@
 C1, C3 and C5 are bench.py's own measurements.  Every line carries its
 parity check: the library's closed-form self-check (the reference binary's
 golden), exact counts, or the same tree alone and in the mix.  Usage:
-  python suite.py [c1 c2 c3 c4 c5]
+  python suite.py [--no-probe] [c1 c2 c3 c4 c5]
+--no-probe skips the floor probe (a child process), for runs under a
+profiler; the floor fields are then left out (C3 needs them).
 """
 from __future__ import annotations
 
@@ -49,7 +51,7 @@ def c2(dev, terms):
             ep_us = r.kernel_ms * 1e3 / r.episodes
             line = {"episodes_per_s": r.episodes / (r.roi_ms / 1e3), "episode_us": ep_us,
                     "parity_ok": bool(np.array_equal(s, want)) and r.state_ok == 1}
-            if name in bench.BARRIERS and not r.sync_scope_used:
+            if terms and name in bench.BARRIERS and not r.sync_scope_used:
                 fl = bench.barrier_floor_us(name, terms)
                 line.update(floor_us=fl, frac_of_floor=fl / ep_us)
             out[f"{name}_{form}{'' if r.sync_scope_used == sc else '_fallback'}"] = line
@@ -111,8 +113,12 @@ def c5(dev, terms):
 
 
 def main(argv):
+    probe = "--no-probe" not in argv
+    argv = [x for x in argv if x != "--no-probe"]
     which = argv or ["c1", "c2", "c3", "c4", "c5"]
-    terms = bench.probe_floor(1024, 256, 10)
+    if not probe and "c3" in which:
+        raise SystemExit("suite.py: c3 needs the floor probe")
+    terms = bench.probe_floor(1024, 256, 10) if probe else None
     dev = hs.Device(0)
     out = {"floor_terms": terms}
     print(json.dumps(out), flush=True)
