@@ -158,7 +158,7 @@ def extra_barriers(dev, hs, a, nw, floor_terms):
             line = {"episodes_per_s": round(r.episodes / (r.roi_ms / 1e3), 1),
                     "episode_us": round(ep_us, 3),
                     "parity_ok": bool(np.array_equal(s, want)) and r.state_ok == 1}
-            if not r.sync_scope_used:
+            if floor_terms and not r.sync_scope_used:
                 fl = barrier_floor_us(name, floor_terms)
                 line.update(floor_us=round(fl, 3), frac_of_floor=round(fl / ep_us, 4))
             out[key] = line
@@ -235,18 +235,9 @@ def extra_c3(dev, hs, floor_terms):
         plain_ms += r.roi_ms
         plain_acq += r.cs_entries + r.rd_entries
         plain_ok = plain_ok and bool(np.array_equal(ps, s)) and r.state_ok == 1
-    # critical-path bound: the instances run concurrently, so the launch
-    # takes at least the longest chain of mutually exclusive acquires, each
-    # at least the floor of one hand-off (measured round trips)
-    chains = [(serialized_acquires(p, nw, it, ncu) *
-               batched_acquire_floor_us(p, nw, L, ncu, floor_terms), i)
-              for i, (p, nw, it, L) in enumerate(mix)]
-    bound_us, crit = max(chains)
-    hand = {L: handoff_us(dev, L) for L in sorted({L for *_, L in mix})}
     batched = rb.acquires / (rb.roi_ms / 1e3)
     plain = plain_acq / (plain_ms / 1e3)
-    cp, cnw, cit, cL = mix[crit]
-    return {"workload": "64 seeded instances (seed 3) of {spinMutex, spinMutexEBO, faMutex, "
+    out = {"workload": "64 seeded instances (seed 3) of {spinMutex, spinMutexEBO, faMutex, "
                         "sleepMutex, spinSem1/2/10/120}, numTBs 8-512 x 64 threads, numCSIters "
                         "log-uniform 10-1000, numLdSt in {1,10,100}; one launch, inc mode",
             "acquires": int(rb.acquires),
@@ -255,18 +246,29 @@ def extra_c3(dev, hs, floor_terms):
             "speedup": round(batched / plain, 3),
             "tuned": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned.items()},
             "autotune_s": round(tune_s, 2),
-            "critical_path": {"instance": crit, "prim": hs.prim_name(cp), "num_wgs": cnw,
-                              "num_cs_iters": cit, "num_ldst": cL,
-                              "serialized_acquires": serialized_acquires(cp, cnw, cit, ncu),
-                              "acquire_floor_us": round(batched_acquire_floor_us(
-                                  cp, cnw, cL, ncu, floor_terms), 4),
-                              "achieved_us_per_acquire": round(
-                                  rb.roi_ms * 1e3 / serialized_acquires(cp, cnw, cit, ncu), 4),
-                              "bound_ms": round(bound_us / 1e3, 3),
-                              "frac_of_bound": round(bound_us / 1e3 / rb.roi_ms, 4),
-                              "plain_famutex_us_per_acquire_by_numldst": {
-                                  str(k): round(v, 4) for k, v in hand.items()}},
             "parity_ok": ok and plain_ok and plain_acq == rb.acquires}
+    if not floor_terms:
+        return out
+    # critical-path bound: the instances run concurrently, so the launch
+    # takes at least the longest chain of mutually exclusive acquires, each
+    # at least the floor of one hand-off (measured round trips)
+    chains = [(serialized_acquires(p, nw, it, ncu) *
+               batched_acquire_floor_us(p, nw, L, ncu, floor_terms), i)
+              for i, (p, nw, it, L) in enumerate(mix)]
+    bound_us, crit = max(chains)
+    hand = {L: handoff_us(dev, L) for L in sorted({L for *_, L in mix})}
+    cp, cnw, cit, cL = mix[crit]
+    out["critical_path"] = {
+        "instance": crit, "prim": hs.prim_name(cp), "num_wgs": cnw, "num_cs_iters": cit,
+        "num_ldst": cL, "serialized_acquires": serialized_acquires(cp, cnw, cit, ncu),
+        "acquire_floor_us": round(batched_acquire_floor_us(cp, cnw, cL, ncu, floor_terms), 4),
+        "achieved_us_per_acquire": round(
+            rb.roi_ms * 1e3 / serialized_acquires(cp, cnw, cit, ncu), 4),
+        "bound_ms": round(bound_us / 1e3, 3),
+        "frac_of_bound": round(bound_us / 1e3 / rb.roi_ms, 4),
+        "plain_famutex_us_per_acquire_by_numldst": {
+            str(k): round(v, 4) for k, v in hand.items()}}
+    return out
 
 
 def c5_trees(n=256, seed=9, depth=(6, 14)):
@@ -569,8 +571,6 @@ def main():
         },
     }
     if not a.no_extras:
-        if floor_terms is None:
-            raise SystemExit("bench.py: the extras need the floor probe (drop --no-probe)")
         extras = {}
         if "barriers" in only:
             extras["barriers"] = extra_barriers(dev, hs, a, nw, floor_terms)

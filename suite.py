@@ -15,7 +15,7 @@ parity check: the library's closed-form self-check (the reference binary's
 golden), exact counts, or the same tree alone and in the mix.  Usage:
   python suite.py [--no-probe] [c1 c2 c3 c4 c5]
 --no-probe skips the floor probe (a child process), for runs under a
-profiler; the floor fields are then left out (C3 needs them).
+profiler; the floor fields are then left out.
 """
 from __future__ import annotations
 
@@ -116,8 +116,6 @@ def main(argv):
     probe = "--no-probe" not in argv
     argv = [x for x in argv if x != "--no-probe"]
     which = argv or ["c1", "c2", "c3", "c4", "c5"]
-    if not probe and "c3" in which:
-        raise SystemExit("suite.py: c3 needs the floor probe")
     terms = bench.probe_floor(1024, 256, 10) if probe else None
     dev = hs.Device(0)
     out = {"floor_terms": terms}
