@@ -44,8 +44,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 # and WRITE_SIZE per dispatch, each scaled by what the same counter reads for
 # a 256 MiB copy kernel (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the command
 # lines and raw figures are in profiles/k_barrier_atomicTreeBarrUniq_1024x256.txt.
-# Key: (prim, sync scope, WG size, numLdSt, numCSIters, num WGs, mode).
-MEASURED_TRAFFIC = {("atomicTreeBarrUniq", 0, 256, 10, 1000, 1024, 0): 405270138}
+# Key: (prim, sync scope, WG size, numLdSt, numCSIters, num WGs, mode); mode 0
+# is the reference's copy chain, mode 1 the in-place increment the headline
+# runs (FETCH 89638.1 KiB x 1.99982 + WRITE 226471.1 KiB x 1.000).
+MEASURED_TRAFFIC = {("atomicTreeBarrUniq", 0, 256, 10, 1000, 1024, 0): 405270138,
+                    ("atomicTreeBarrUniq", 0, 256, 10, 1000, 1024, 1): 415468290}
 METRIC = ("acquire+release/s & barrier episodes/s at 1 GPU, batched vs plain; "
           "UTS nodes/s, p99")
 BARRIERS = ("atomicTreeBarrUniq", "atomicTreeBarrUniqLocalExch", "lfTreeBarrUniq",
