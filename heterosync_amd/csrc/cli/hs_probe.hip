@@ -15,6 +15,7 @@
 // hand-off on one XCD and across XCDs, and one address's RMW spacing.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
@@ -202,12 +203,13 @@ __global__ void k_l2pass(int32_t *data, int L, int passes) {
   }
 }
 
-// Dependent chains of one lane, `n` steps each; out[0] keeps the values live.
+// Dependent chains of one lane of WG `who`, `n` steps each; out[0] keeps the
+// values live.
 //   0: agent-scope RMW (fetch_add) whose address depends on the last result
 //   1: agent-scope load whose address depends on the last loaded value
 //   2: agent-scope store followed by a drain of the memory counter
-__global__ void k_chain(int mode, int n, uint32_t *p, uint32_t *out) {
-  if (threadIdx.x != 0) return;
+__global__ void k_chain(int who, int mode, int n, uint32_t *p, uint32_t *out) {
+  if (threadIdx.x != 0 || (int)blockIdx.x != who) return;
   uint32_t v = 0;
   for (int i = 0; i < n; ++i) {
     if (mode == 0) {
@@ -262,25 +264,45 @@ int floor_terms(int nwg, int lanes, int L) {
   CHECK(hipMalloc(&res, 4096));
   const int passes = 200;
   const float pass_ms = time_ms([&] { k_l2pass<<<nwg, lanes>>>(data, L, passes); });
-  double chain_us[3];
+  // Latencies depend on where the two ends sit: XCDs on the same I/O die
+  // hand a line over faster than XCDs on different dies, and a memory-side
+  // RMW's round trip depends on the issuing XCD and the counter's channel.
+  // WG b of a small launch runs on XCD (b + s) mod 8, s set by the launch,
+  // so each term is measured from all 8 XCDs (chains: WG 0..7 in turn) or
+  // to all 7 other XCDs (hops: WG 0 with WG 1..7), and reported as the
+  // mean; the release of a grid barrier must reach every XCD, so the
+  // slowest of the 7 hops is reported as well (hop_far).
+  double chain_us[3] = {0, 0, 0};
   for (int mode = 0; mode < 3; ++mode) {
-    const int n = 4000;
-    CHECK(hipMemset(buf, 0, 1 << 16));
-    const float ms = time_ms([&] { k_chain<<<1, 64>>>(mode, n, buf, res); });
-    chain_us[mode] = ms * 1000 / n;
+    const int n = 2000;
+    for (int who = 0; who < 8; ++who) {
+      CHECK(hipMemset(buf, 0, 1 << 16));
+      const float ms = time_ms([&] { k_chain<<<8, 64>>>(who, mode, n, buf, res); });
+      chain_us[mode] += ms * 1000 / n / 8;
+    }
   }
-  double hop_us[2];
-  for (int k = 0; k < 2; ++k) {  // WG 8 shares WG 0's XCD, WG 1 does not
-    const int rounds = 4000, other = k == 0 ? 8 : 1;
+  auto hop = [&](int a, int b, double *us) {
+    const int rounds = 2000;
     CHECK(hipMemset(res, 0, 64));
     const float ms = time_ms([&] {
       CHECK(hipMemset(buf, 0, 256));
-      k_pingpong<<<16, 64>>>(0, other, rounds, buf, res);
+      k_pingpong<<<16, 64>>>(a, b, rounds, buf, res);
     });
     uint32_t hr[4];
     CHECK(hipMemcpy(hr, res, 16, hipMemcpyDeviceToHost));
-    if (hr[2]) return 2;
-    hop_us[k] = ms * 1000 / (2 * rounds);
+    if (hr[2]) return false;
+    *us = ms * 1000 / (2 * rounds);
+    return true;
+  };
+  double hop_same = 0, hop_other = 0, hop_far = 0, us = 0;
+  for (int j = 0; j < 8; ++j) {  // WG j and WG j + 8 share an XCD
+    if (!hop(j, j + 8, &us)) return 2;
+    hop_same += us / 8;
+  }
+  for (int j = 1; j < 8; ++j) {  // WG 0 and WG j do not
+    if (!hop(0, j, &us)) return 2;
+    hop_other += us / 7;
+    hop_far = std::max(hop_far, us);
   }
   CHECK(hipMemset(buf, 0, 1 << 16));
   const int it = 1000;
@@ -289,9 +311,9 @@ int floor_terms(int nwg, int lanes, int L) {
   std::printf("{\"wgs\": %d, \"lanes\": %d, \"num_ldst\": %d, \"l2_pass_us\": %.4f, "
               "\"l2_pass_gbs\": %.1f, \"rmw_rt_us\": %.4f, \"load_rt_us\": %.4f, "
               "\"store_rt_us\": %.4f, \"hop_same_xcd_us\": %.4f, \"hop_other_xcd_us\": %.4f, "
-              "\"same_address_rmw_ns\": %.3f}\n",
+              "\"hop_far_xcd_us\": %.4f, \"same_address_rmw_ns\": %.3f}\n",
               nwg, lanes, L, pass_ms * 1000 / passes, bytes * passes / (pass_ms * 1e6),
-              chain_us[0], chain_us[1], chain_us[2], hop_us[0], hop_us[1],
+              chain_us[0], chain_us[1], chain_us[2], hop_same, hop_other, hop_far,
               one_ms * 1e6 / ((double)it * 1024));
   CHECK(hipFree(data));
   CHECK(hipFree(buf));

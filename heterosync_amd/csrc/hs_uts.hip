@@ -1406,7 +1406,42 @@ int run_single(hs_ctx *c, Engine &e, uint32_t seed, uint32_t gen_mx, long depth,
   return e.read_slot(c->stream, 0, res);
 }
 
-void free_engine(void *p) { delete static_cast<Engine *>(p); }
+// The geometric engines a context keeps: the two most recently used grids.
+// The default grid depends on the tree's expected size (all resident WGs
+// for large trees, 8 per CU otherwise), so a stream of mixed trees
+// alternates between two grids; one cached engine would be freed and
+// rebuilt (GBs of deques) at every alternation.
+struct EngineCache {
+  static constexpr int kEntries = 2;
+  Engine *e[kEntries] = {};
+  ~EngineCache() {
+    for (Engine *p : e) delete p;
+  }
+  // The engine for (nb, b0), most recently used first; built on a miss,
+  // replacing the least recently used.  nullptr and *rc on failure.
+  Engine *get(hs_ctx *c, int nb, double b0, int *rc) {
+    int i = 0;
+    while (i < kEntries && !(e[i] && e[i]->matches(1, nb, kDefaultDeque, b0))) ++i;
+    Engine *hit = nullptr;
+    if (i < kEntries) {
+      hit = e[i];
+    } else {
+      i = kEntries - 1;
+      delete e[i];
+      e[i] = nullptr;
+      hit = new Engine;
+      if ((*rc = hit->setup(c, 1, nb, kDefaultDeque, b0))) {
+        delete hit;
+        return nullptr;
+      }
+    }
+    for (; i > 0; --i) e[i] = e[i - 1];
+    e[0] = hit;
+    return hit;
+  }
+};
+
+void free_engines(void *p) { delete static_cast<EngineCache *>(p); }
 
 }  // namespace
 
@@ -1464,9 +1499,9 @@ int uts_ref(hs_ctx *c, int num_nodes, const uint32_t *value, const int32_t *chil
   return run_single<0>(c, e, 0, 0, height, res, cpu_threads, host, cpu);
 }
 
-// The geometric engine of a context is kept between calls (same grid, deque
-// size and b0), so a call costs a state fill, the root delivery, the kernel
-// and one small copy back.
+// The geometric engines of a context are kept between calls (same grid,
+// deque size and b0; the two most recent grids), so a call costs a state
+// fill, the root delivery, the kernel and one small copy back.
 int uts_geo(hs_ctx *c, int seed, double b0, int gen_mx, int num_blocks, int cpu_threads,
             hs_uts_result *res, hs_uts_cpu_result *cpu) {
   if (!c || !res || !(b0 > 0 && b0 < 1000) || gen_mx < 0 || gen_mx > 60000 || num_blocks < 0 ||
@@ -1475,20 +1510,13 @@ int uts_geo(hs_ctx *c, int seed, double b0, int gen_mx, int num_blocks, int cpu_
   std::memset(res, 0, sizeof *res);
   HS_HIP(hipSetDevice(c->device));
   const int nb = num_blocks > 0 ? num_blocks : Engine::default_blocks(c, b0, gen_mx);
-  Engine *e = static_cast<Engine *>(c->uts_cache);
-  if (!e || !e->matches(1, nb, kDefaultDeque, b0)) {
-    if (c->uts_cache_free) c->uts_cache_free(c->uts_cache);
-    c->uts_cache = nullptr;
-    c->uts_cache_free = nullptr;
-    e = new Engine;
-    int rc = e->setup(c, 1, nb, kDefaultDeque, b0);
-    if (rc) {
-      delete e;
-      return rc;
-    }
-    c->uts_cache = e;
-    c->uts_cache_free = free_engine;
+  if (!c->uts_cache) {
+    c->uts_cache = new EngineCache;
+    c->uts_cache_free = free_engines;
   }
+  int rc = HS_OK;
+  Engine *e = static_cast<EngineCache *>(c->uts_cache)->get(c, nb, b0, &rc);
+  if (!e) return rc;
   if (cpu_threads > e->blocks) return HS_EINVAL;  // one serving WG per host worker
   HostTree host;
   host.tree = 1;
@@ -1534,15 +1562,16 @@ extern "C" int hs_uts_serve(hs_ctx *c, const hs_uts_request *req, int n, const h
   std::memset(resp, 0, sizeof(*resp) * n);
   const int slots = cfg->slots > 0 ? std::min(cfg->slots, kMaxSlots) : 8;
   if (cfg->num_blocks < 0 || cfg->deque_nodes < 0) return HS_EINVAL;
-  const int blocks = cfg->num_blocks > 0 ? cfg->num_blocks : 8 * c->num_cu;
-  if (blocks < 2) return HS_EINVAL;  // the dispatcher and at least one worker
   HS_HIP(hipSetDevice(c->device));
-  int rc;
-  {  // persistent: every WG, the dispatcher (the grid's last) included, must be resident
-    int fit = 0;
-    if ((rc = coresident_wgs(c, (const void *)k_uts_serve, 64, &fit))) return rc;
-    if (blocks > fit) return HS_EOCCUPANCY;
-  }
+  int rc, fit = 0;
+  // persistent: every WG, the dispatcher (the grid's last) included, must be
+  // resident.  The default is every WG that can be (three waves per SIMD at
+  // the kernel's register use): on the 256-tree depth 6-14 mix it served
+  // 38-43 G nodes/s against 35-37 G with 8 WGs per CU.
+  if ((rc = coresident_wgs(c, (const void *)k_uts_serve, 64, &fit))) return rc;
+  const int blocks = cfg->num_blocks > 0 ? cfg->num_blocks : fit;
+  if (blocks < 2) return HS_EINVAL;  // the dispatcher and at least one worker
+  if (blocks > fit) return HS_EOCCUPANCY;
   Engine e;
   const int cap = cfg->deque_nodes > 0 ? cfg->deque_nodes : kDefaultDeque;
   if ((rc = e.setup(c, 1, blocks, cap, b0))) return rc;

@@ -235,7 +235,9 @@ int hs_uts_ref_run(hs_ctx *ctx, int num_nodes, const uint32_t *value,
  * per SIMD) for trees expected above 2.5e8 nodes.  Deques hold 2^16
  * records; HS_EINVAL when gen_mx * (max_children - 1) exceeds that (the
  * deque bound, see hs_uts_serve: gen_mx <= 689 at b0 = 4, whose nodes
- * have at most 96 children). */
+ * have at most 96 children).  The context keeps the engines (deques,
+ * inboxes, state; about 1.5 MB per WG) of the two most recent grids
+ * between calls; hs_close frees them. */
 int hs_uts_geo_run(hs_ctx *ctx, int seed, double b0, int gen_mx,
                    int num_blocks, hs_uts_result *res);
 
@@ -339,8 +341,9 @@ typedef struct {
 typedef struct {
   int slots;          /* concurrent trees on the device (the response
                          board); 0 = 8, at most 64 */
-  int num_blocks;     /* persistent WGs including the dispatcher; 0 = 8 per
-                         CU (two waves per SIMD).  HS_EOCCUPANCY if they
+  int num_blocks;     /* persistent WGs including the dispatcher; 0 = every
+                         WG that can be resident (three waves per SIMD at
+                         the kernel's register use).  HS_EOCCUPANCY if they
                          cannot all be resident. */
   int deque_nodes;    /* records per WG deque (HBM); 0 = 65536 */
   double timeout_s;

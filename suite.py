@@ -8,7 +8,8 @@
   C3  64-instance mutex/semaphore mix: batched persistent launch vs plain
   C4  relaxed atomics 1024 WGs x 256 lanes x 10^4 iterations
   C5  UTS serving loop vs one launch per tree (geometric SHA-1 trees, depth
-      limit U[6,14]), at 16 and 64 slots
+      limit U[6,14]), at 64 and 16 slots; Poisson arrivals at 0.5, 0.8 and
+      0.95 x the plain form's capacity (64 slots)
 
 C1, C3 and C5 are bench.py's own measurements.  Every line carries its
 parity check: the library's closed-form self-check (the reference binary's
@@ -108,8 +109,8 @@ def c4(dev, terms, iters=10000):
 
 
 def c5(dev, terms):
-    return {"slots_16": bench.extra_c5(dev, slots=16),
-            "slots_64": bench.extra_c5(dev, slots=64)}
+    return {"slots_64": bench.extra_c5(dev, slots=64, loads=(0.8, 0.5, 0.95)),
+            "slots_16": bench.extra_c5(dev, slots=16)}
 
 
 def main(argv):

@@ -80,8 +80,9 @@ def test_uts_serve_alone_equals_mixed(dev, oracle):
 def test_uts_serve_rejects_grid_that_cannot_be_resident(dev):
     """The serving loop is persistent (its dispatcher is the grid's last WG),
     so a grid larger than what can be co-resident is refused before launch.
-    The default grid (8 WGs per CU) is checked the same way: it runs, and so
-    does the same grid given explicitly."""
+    The default grid is every WG that can be resident (at least 8 per CU at
+    the kernel's register use): it runs, and so do 8 per CU given
+    explicitly."""
     import heterosync_amd as hs
     with pytest.raises(hs.HSError) as ei:
         dev.uts_serve([(1, 3, 4.0, 0.0)], slots=2, num_blocks=64 * dev.info.num_cu)
