@@ -204,18 +204,23 @@ def serialized_acquires(p, nw, iters, ncu):
     return writers * iters
 
 
-def batched_acquire_floor_us(p, nw, L, ncu, t):
+def batched_acquire_floor_us(p, nw, L, ncu, t, cohort=1):
     """Lower bound on one exclusive acquire+release of a batched instance:
     the release is seen by the next holder (a hand-off across XCDs), which
     makes its CAS where the lock needs one (spin, EBO, semaphores), loads
     its lines (at least one round trip per 64 lines: the widest worker
     splits a chain over four waves, 16 lines per load batch) and drains its
     stores (one store round trip) before releasing.  A semaphore writer
-    walks its group's perCU regions."""
+    walks its group's perCU regions.  With the mutex cohort a spin / EBO
+    lock hold serves up to `cohort` critical sections on one worker, and
+    only the first pays the hand-off and the CAS: the bound takes every
+    hold at its fullest."""
     regions = 1 if p <= 7 else -(-nw // min(nw, ncu))
     cas = t["rmw_rt_us"] if p in (4, 5) or p >= 8 else 0.0
     cs = max(1, -(-L // 64)) * t["load_rt_us"] * regions + t["store_rt_us"]
-    return t["hop_other_xcd_us"] + cas + cs
+    handoff = t["hop_other_xcd_us"] + cas + cs
+    c = max(1, cohort) if p in (4, 5) else 1
+    return (handoff + (c - 1) * cs) / c
 
 
 def extra_c3(dev, hs, floor_terms):
@@ -256,9 +261,11 @@ def extra_c3(dev, hs, floor_terms):
         return out
     # critical-path bound: the instances run concurrently, so the launch
     # takes at least the longest chain of mutually exclusive acquires, each
-    # at least the floor of one hand-off (measured round trips)
+    # at least the floor of one hand-off (measured round trips), or of one
+    # cohort critical section where the tuned run uses the cohort
+    coh = tuned.get("cohort", 0)
     chains = [(serialized_acquires(p, nw, it, ncu) *
-               batched_acquire_floor_us(p, nw, L, ncu, floor_terms), i)
+               batched_acquire_floor_us(p, nw, L, ncu, floor_terms, coh), i)
               for i, (p, nw, it, L) in enumerate(mix)]
     bound_us, crit = max(chains)
     hand = {L: handoff_us(dev, L) for L in sorted({L for *_, L in mix})}
@@ -266,7 +273,9 @@ def extra_c3(dev, hs, floor_terms):
     out["critical_path"] = {
         "instance": crit, "prim": hs.prim_name(cp), "num_wgs": cnw, "num_cs_iters": cit,
         "num_ldst": cL, "serialized_acquires": serialized_acquires(cp, cnw, cit, ncu),
-        "acquire_floor_us": round(batched_acquire_floor_us(cp, cnw, cL, ncu, floor_terms), 4),
+        "acquire_floor_us": round(
+            batched_acquire_floor_us(cp, cnw, cL, ncu, floor_terms, coh), 4),
+        "cohort": coh,
         "achieved_us_per_acquire": round(
             rb.roi_ms * 1e3 / serialized_acquires(cp, cnw, cit, ncu), 4),
         "bound_ms": round(bound_us / 1e3, 3),

@@ -87,7 +87,7 @@ class BatchCfg(ctypes.Structure):
                 ("backoff_min", ctypes.c_int), ("backoff_max", ctypes.c_int),
                 ("poll_sleep", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("persistent_wgs", ctypes.c_int), ("timeout_s", ctypes.c_double),
-                ("probe", ctypes.c_int)]
+                ("probe", ctypes.c_int), ("cohort", ctypes.c_int)]
 
 
 class BatchResult(ctypes.Structure):
@@ -372,33 +372,35 @@ class Device:
     # ---- batched ---------------------------------------------------------
     def batch_autotune(self, instances, reps=10, mode=0, use_graph=1, timeout_s=20.0):
         """Returns the tuned knobs as a dict: wg_size, persistent_wgs,
-        poll_sleep, backoff_min, backoff_max, and best_ms (the winning
-        calibration launch).  batch(**{k: v for k in TUNED}) runs them."""
+        poll_sleep, backoff_min, backoff_max, probe, cohort, and best_ms (the
+        winning calibration launch).  batch(**{k: v for k in TUNED}) runs them."""
         n = len(instances)
         inst = (BatchInstance * n)(*[BatchInstance(*i) for i in instances])
-        cfg = BatchCfg(64, reps, mode, 0, 0, 0, use_graph, 0, timeout_s, 0)
+        cfg = BatchCfg(64, reps, mode, 0, 0, 0, use_graph, 0, timeout_s, 0, 0)
         best = ctypes.c_double()
         _check(_lib().hs_batch_autotune(self._h, inst, n, ctypes.byref(cfg), ctypes.byref(best)),
                "hs_batch_autotune")
         return {"wg_size": cfg.wg_size, "persistent_wgs": cfg.persistent_wgs,
                 "poll_sleep": cfg.poll_sleep, "backoff_min": cfg.backoff_min,
-                "backoff_max": cfg.backoff_max, "probe": cfg.probe, "best_ms": best.value}
+                "backoff_max": cfg.backoff_max, "probe": cfg.probe, "cohort": cfg.cohort,
+                "best_ms": best.value}
 
     @staticmethod
     def tuned_kwargs(t):
         """batch() keyword arguments for a batch_autotune() result."""
         return dict(wg_size=t["wg_size"], persistent_wgs=t["persistent_wgs"],
                     poll_sleep=t["poll_sleep"], backoff=(t["backoff_min"], t["backoff_max"]),
-                    probe=t["probe"])
+                    probe=t["probe"], cohort=t["cohort"])
 
     def batch(self, instances, storages, wg_size=64, reps=10, mode=0, backoff=(0, 0),
-              poll_sleep=0, use_graph=1, persistent_wgs=0, timeout_s=20.0, probe=0):
+              poll_sleep=0, use_graph=1, persistent_wgs=0, timeout_s=20.0, probe=0,
+              cohort=0):
         n = len(instances)
         inst = (BatchInstance * n)(*[BatchInstance(*i) for i in instances])
         ptrs = (ctypes.POINTER(ctypes.c_int32) * n)(*[_ptr(s, ctypes.c_int32) for s in storages])
         entries = np.zeros(n, np.int64)
         cfg = BatchCfg(wg_size, reps, mode, backoff[0], backoff[1], poll_sleep, use_graph,
-                       persistent_wgs, timeout_s, probe)
+                       persistent_wgs, timeout_s, probe, cohort)
         res = BatchResult()
         _check(_lib().hs_batch_run(self._h, inst, n, ctypes.byref(cfg), ptrs,
                                    _ptr(entries, ctypes.c_int64), ctypes.byref(res)),

@@ -5,8 +5,10 @@
 // Each instance keeps the plain form's semantics exactly (the kernels of
 // hip/syncPrims/uvm/main.hip.cpp:645-930, 1147-1214): numWGs "virtual" WGs
 // of 64 lanes, each doing numCSIters {acquire; critical-section walk;
-// release}.  A fixed grid of co-resident wave64 workers multiplexes the
-// virtual WGs (static round-robin assignment).  A worker never blocks on one
+// release}.  A fixed grid of co-resident workers of 1, 2 or 4 waves
+// multiplexes the virtual WGs (static assignment: round robin, or, with the
+// mutex cohort, each spin / EBO mutex instance in consecutive slots).  A
+// worker never blocks on one
 // virtual WG: it makes one non-blocking attempt (TTAS, ticket check, queue
 // slot check or semaphore CAS) and moves on, so a spinning reader can never
 // starve the writer it waits for, whatever the residency.  Backoff becomes
@@ -42,6 +44,8 @@ struct BInst {
 struct BArgs {
   const BInst *inst;
   int ninst, nv, vpw;
+  const int *vmap;  // worker w's slot j holds virtual WG vmap[w * vpw + j] (-1: none)
+  int cohort;       // spin / EBO mutexes: most critical sections per lock hold
   uint32_t *ctl;
   uint64_t ticks;
   Backoff bo;
@@ -146,15 +150,15 @@ __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
   __shared__ int s_remaining, s_enter, s_ok;
   __shared__ unsigned long long s_cand, s_sure;
   __shared__ int32_t sink[64 * S];
-  const int lane = threadIdx.x, P = gridDim.x, w = blockIdx.x;
+  const int lane = threadIdx.x, w = blockIdx.x;
   Guard g;
   g.init(a.ctl, a.ticks);
   sink[lane] = 0;
   if (lane == 0) {
     int rem = 0;
     for (int j = 0; j < a.vpw; ++j) {
-      const int v = w + j * P;
-      if (v < a.nv) {
+      const int v = a.vmap[w * a.vpw + j];
+      if (v >= 0 && v < a.nv) {
         const int i = find_inst(a.inst, a.ninst, v);
         s_inst[j] = i;
         s_b[j] = v - a.inst[i].vbase;
@@ -243,8 +247,12 @@ __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
     const unsigned long long sure = s_sure;
     int progress = 0;
     while (cand) {
-      const int j = (int)__builtin_ctzll(cand);
-      cand &= cand - 1;
+      // virtual WGs whose turn has come (ticket / queue lock) first: their
+      // instance waits for them, whichever worker they are on
+      const unsigned long long first = cand & sure;
+      int j = (int)__builtin_ctzll(first ? first : cand);
+      cand &= ~(1ull << j);
+      if (s_phase[j] == kDone) continue;  // finished by a cohort hand-off this pass
       const BInst &I = a.inst[s_inst[j]];
       const int p = I.prim;
       const bool sem = p >= 8;
@@ -298,33 +306,81 @@ __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
       }
       __syncthreads();
       if (!s_enter) continue;
-      // ---- critical section of virtual WG b of instance I --------------
-      if (!sem) {
-        vwg_access<MODE, S>(I.storage, 0, I.L);
-      } else if (writer) {
-        for (int k = 0; k < I.per; ++k) vwg_access<MODE, S>(I.storage, (long)k * 64, I.L);
-      } else {
-        vwg_read<S>(I.storage, (long)pw * 64, I.L, sink);
-      }
-      wait_mem();  // every lane's CS accesses complete before the release
-      __syncthreads();
-      if (lane == 0) {
-        uint32_t *lk = I.lk;
-        if (!sem) {
-          if (p == 4 || p == 5) mutex_unlock<kSpin, true>(lk, nullptr, 0, 0);
-          else if (p == 7) mutex_unlock<kFA, true>(lk, nullptr, 0, s_ticket[j]);
-          else mutex_unlock<kSleep, true>(lk, I.ring, I.ring_stride, s_ticket[j]);
-          add_rlx(lk + kLkCs, 1);  // entry count, off the hand-off path
+      // Virtual WG j holds the lock or has entered the semaphore.  A spin or
+      // EBO mutex may pass straight on to another of this worker's virtual
+      // WGs of the same instance (cohort hand-off): the lock word stays
+      // held, which is the outcome of that WG winning the CAS right after
+      // this one's release; mutual exclusion and the per-WG iteration
+      // counts are unchanged.  A ticket or queue lock releases as always,
+      // and if the next ticket belongs to one of this worker's virtual WGs,
+      // that WG enters at once instead of at the worker's next probe (its
+      // turn has come; the order stays the lock's).
+      for (int chain = 1;; ++chain) {
+        const BInst &H = a.inst[s_inst[j]];
+        const int hp = H.prim;
+        const bool hsem = hp >= 8;
+        const int hb = s_b[j];
+        const int hgrp = hb % H.nab, hpw = hb / H.nab;
+        const bool hwriter = !hsem || hpw == hgrp % H.per;  // main.hip.cpp:883
+        // ---- critical section of virtual WG hb of instance H ------------
+        if (!hsem) {
+          vwg_access<MODE, S>(H.storage, 0, H.L);
+        } else if (hwriter) {
+          for (int k = 0; k < H.per; ++k) vwg_access<MODE, S>(H.storage, (long)k * 64, H.L);
         } else {
-          sem_post<false, true>(lk, writer ? I.sem_max : 1u);
-          add_rlx(lk + (writer ? kLkCs : kLkRd), 1);
+          vwg_read<S>(H.storage, (long)hpw * 64, H.L, sink);
         }
-        s_bo[j] = a.bo.min_q;
-        s_phase[j] = ++s_iter[j] == I.iters ? kDone : kIdle;
-        if (s_phase[j] == kDone) --s_remaining;
+        wait_mem();  // every lane's CS accesses complete before the release
+        __syncthreads();
+        int nxt = -1;
+        if ((hp == 4 || hp == 5) && chain < a.cohort) {
+          if (lane < 64) {
+            const bool want = lane < a.vpw && lane != j && s_phase[lane] != kDone &&
+                              s_inst[lane] == s_inst[j];
+            const unsigned long long m = __ballot(want);
+            if (lane == 0) {
+              // the next waiting virtual WG after j, cyclically
+              const unsigned long long hi = j < 63 ? m & (~0ull << (j + 1)) : 0;
+              s_enter = m == 0 ? -1 : (int)__builtin_ctzll(hi ? hi : m);
+            }
+          }
+          __syncthreads();
+          nxt = s_enter;
+        } else if (hp == 6 || hp == 7) {
+          if (lane < 64) {
+            const bool next = lane < a.vpw && lane != j && s_phase[lane] == kWaiting &&
+                              s_inst[lane] == s_inst[j] && s_ticket[lane] == s_ticket[j] + 1;
+            const unsigned long long m = __ballot(next);
+            if (lane == 0) s_enter = m == 0 ? -1 : (int)__builtin_ctzll(m);
+          }
+          __syncthreads();
+          nxt = s_enter;
+        }
+        if (lane == 0) {
+          uint32_t *lk = H.lk;
+          if (!hsem) {
+            if (hp == 4 || hp == 5) {
+              if (nxt < 0) mutex_unlock<kSpin, true>(lk, nullptr, 0, 0);
+            } else if (hp == 7) {
+              mutex_unlock<kFA, true>(lk, nullptr, 0, s_ticket[j]);
+            } else {
+              mutex_unlock<kSleep, true>(lk, H.ring, H.ring_stride, s_ticket[j]);
+            }
+            add_rlx(lk + kLkCs, 1);  // entry count, off the hand-off path
+          } else {
+            sem_post<false, true>(lk, hwriter ? H.sem_max : 1u);
+            add_rlx(lk + (hwriter ? kLkCs : kLkRd), 1);
+          }
+          s_bo[j] = a.bo.min_q;
+          s_skip[j] = 0;
+          s_phase[j] = ++s_iter[j] == H.iters ? kDone : kIdle;
+          if (s_phase[j] == kDone) --s_remaining;
+        }
+        progress = 1;
+        __syncthreads();
+        if (nxt < 0) break;
+        j = nxt;
       }
-      progress = 1;
-      __syncthreads();
     }
     if (lane == 0 && !progress) {
       if (!g.ok()) s_ok = 0;
@@ -441,6 +497,46 @@ BatchFn pick_batch(const hs_batch_cfg *cfg) {
   return cfg->probe ? pick_batch_t<true>(cfg->mode, waves) : pick_batch_t<false>(cfg->mode, waves);
 }
 
+// Places the virtual WGs on the workers' slots (worker w's slot j holds
+// map[w * vpw + j]): round robin (worker v % P, slot v / P), or, with the
+// cohort, each mutex instance's WGs in consecutive slots, filling one
+// worker after another.  A mutex serializes its instance wherever its WGs
+// run, and in blocks a spin / EBO lock holder finds the instance's other
+// waiters on its own worker, and a worker's WGs take consecutive tickets of
+// a ticket or queue lock (one wave-wide fetch-add), so the next ticket's
+// owner is usually local.  Semaphores stay round robin over the free
+// slots: their readers run concurrently only on different workers.
+std::vector<int> place_vwgs(const Batch &B, int P, int vpw, bool blocks) {
+  std::vector<int> map((size_t)P * vpw, -1);
+  if (!blocks) {
+    for (int v = 0; v < B.nv; ++v) map[(size_t)(v % P) * vpw + v / P] = v;
+    return map;
+  }
+  std::vector<int> fill(P, 0);
+  int w = 0;
+  for (const BInst &I : B.h) {
+    if (I.prim > 7) continue;  // semaphores below
+    for (int k = 0; k < I.nw; ++k) {
+      while (w < P && fill[w] == vpw) ++w;
+      if (w == P) return {};
+      map[(size_t)w * vpw + fill[w]++] = I.vbase + k;
+    }
+  }
+  int rr = 0;
+  for (const BInst &I : B.h) {
+    if (I.prim <= 7) continue;
+    for (int k = 0; k < I.nw; ++k) {
+      for (int tries = 0; fill[rr] == vpw; ++tries) {
+        if (tries == P) return {};
+        rr = (rr + 1) % P;
+      }
+      map[(size_t)rr * vpw + fill[rr]++] = I.vbase + k;
+      rr = (rr + 1) % P;
+    }
+  }
+  return map;
+}
+
 // Runs `reps` persistent launches (optionally as one captured graph) and
 // returns the ROI in ms.
 int batch_launch(hs_ctx *c, Batch &B, int n, const hs_batch_cfg *cfg, int reps, int workers,
@@ -455,13 +551,22 @@ int batch_launch(hs_ctx *c, Batch &B, int n, const hs_batch_cfg *cfg, int reps, 
   const int P = std::max(1, std::min(workers, B.nv));
   a.vpw = (B.nv + P - 1) / P;
   if (a.vpw > kMaxVpw) return HS_EINVAL;
+  int rc;
+  {
+    const std::vector<int> map = place_vwgs(B, P, a.vpw, cfg->cohort > 1);
+    if (map.empty()) return HS_EINVAL;
+    int *d_map = nullptr;
+    if ((rc = B.ar.alloc(&d_map, map.size()))) return rc;
+    HS_HIP(hipMemcpy(d_map, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice));
+    a.vmap = d_map;
+  }
+  a.cohort = cfg->cohort > 1 ? cfg->cohort : 1;
   a.ctl = B.d_ctl;
   a.ticks = timeout_ticks(c, cfg->timeout_s);
   a.bo.min_q = cfg->backoff_min > 0 ? cfg->backoff_min : 1;
   a.bo.max_q = cfg->backoff_max > 0 ? cfg->backoff_max : 1023;
   a.bo.poll_q = cfg->poll_sleep < 0 ? 0 : (cfg->poll_sleep > 0 ? cfg->poll_sleep : 1);
   EventSet ev;
-  int rc;
   if ((rc = ev.create(2))) return rc;
   struct Graph {  // destroyed on every return path
     hipGraph_t g = nullptr;
@@ -524,6 +629,7 @@ extern "C" int hs_batch_run(hs_ctx *c, const hs_batch_instance *in, int n,
   if (worker_waves(cfg) == 0) return HS_EINVAL;  // workers of 64, 128 or 256 lanes
   if (cfg->probe != 0 && cfg->probe != 1) return HS_EINVAL;
   if (cfg->mode != 0 && cfg->mode != 1) return HS_EINVAL;
+  if (cfg->cohort < 0) return HS_EINVAL;
   std::memset(res, 0, sizeof *res);
   const int reps = cfg->reps > 0 ? cfg->reps : 10;
   HS_HIP(hipSetDevice(c->device));
@@ -571,6 +677,7 @@ extern "C" int hs_batch_autotune(hs_ctx *c, const hs_batch_instance *in, int n,
   best_cfg.wg_size = 64;
   best_cfg.backoff_min = 1;
   best_cfg.probe = 0;
+  best_cfg.cohort = 0;
   int best_wpc = 8;
   // One calibration launch of t; *ms < 0 when its workers cannot all be resident.
   auto trial = [&](const hs_batch_cfg &t, double *ms) -> int {
@@ -634,6 +741,21 @@ extern "C" int hs_batch_autotune(hs_ctx *c, const hs_batch_instance *in, int n,
         best_cfg = t;
       }
     }
+  // last, the spin / EBO mutex cohort at the winning knobs
+  {
+    const hs_batch_cfg base2 = best_cfg;
+    for (int coh : {8, 64}) {
+      hs_batch_cfg t = base2;
+      t.cohort = coh;
+      double ms = 0;
+      const int rc = trial(t, &ms);
+      if (rc) return rc;
+      if (ms >= 0 && ms < best) {
+        best = ms;
+        best_cfg = t;
+      }
+    }
+  }
   best_cfg.reps = cfg->reps;
   *cfg = best_cfg;
   if (best_ms) *best_ms = best;

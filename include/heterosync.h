@@ -298,6 +298,18 @@ typedef struct {
                         turn (one round trip each), 1 = one load per lane
                         probes all of a worker's virtual WGs at once, then
                         lane 0 acts on the hopeful ones */
+  int cohort;        /* spin and EBO mutexes: most critical sections per
+                        hold of the lock word.  A worker that holds an
+                        instance's lock passes it to another of its virtual
+                        WGs of that instance still waiting, without the
+                        release store and the next CAS (the same outcome as
+                        that WG winning the CAS right after a release), up to
+                        `cohort` critical sections, then releases.  Every
+                        mutex instance's virtual WGs are then placed in
+                        blocks on workers instead of round robin (a ticket
+                        or queue lock's next ticket is then usually on the
+                        releasing worker, whose WG enters at once).  0 or
+                        1 = off (every critical section releases). */
 } hs_batch_cfg;
 
 typedef struct {
@@ -319,9 +331,10 @@ int hs_batch_run(hs_ctx *ctx, const hs_batch_instance *inst, int n,
  * {2,4,8,16} x poll interval {0,1,4} x EBO cap I_max {63,255,1023} with
  * 64-lane workers, then, at the winning waves per CU, worker width
  * {64,128,256} x backoff start I_min {1,4,16} x attempt form (probe 0,
- * 1).  Writes the fastest wg_size / persistent_wgs / poll_sleep /
- * backoff_min / backoff_max / probe into *cfg (grids that cannot be
- * co-resident are skipped). */
+ * 1), and last the mutex cohort {off, 8, 64} at the winning knobs.
+ * Writes the fastest wg_size / persistent_wgs / poll_sleep / backoff_min /
+ * backoff_max / probe / cohort into *cfg (grids that cannot be co-resident
+ * are skipped). */
 int hs_batch_autotune(hs_ctx *ctx, const hs_batch_instance *inst, int n,
                       hs_batch_cfg *cfg, double *best_ms);
 

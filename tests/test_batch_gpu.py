@@ -60,7 +60,7 @@ def test_batch_full_mix_inc_autotuned(dev, ohelp):
     mix = make_mix(seed=3, n=64)
     t = dev.batch_autotune(mix, reps=1, mode=1)
     assert t["wg_size"] in (64, 128, 256) and t["backoff_min"] in (1, 4, 16)
-    assert t["probe"] in (0, 1)
+    assert t["probe"] in (0, 1) and t["cohort"] in (0, 8, 64)
     assert t["persistent_wgs"] > 0 and t["best_ms"] > 0
     for _ in range(2):
         st = storages_for(dev, mix)
@@ -100,6 +100,35 @@ def test_batch_mix_seeds_and_workers(dev, ohelp, seed, workers, probe):
     check_mix(dev, ohelp, mix, st, entries, 2, 1)
 
 
+@pytest.mark.parametrize("probe", [0, 1])
+@pytest.mark.parametrize("cohort", [8, 64])
+def test_batch_full_mix_inc_cohort(dev, ohelp, cohort, probe):
+    """The full mix with the spin / EBO mutex cohort: a lock holder passes
+    the lock to its worker's other waiting virtual WGs of the instance (up
+    to `cohort` critical sections per hold).  Inc mode, so a critical
+    section that overlapped another or ran twice changes the storage; every
+    instance still makes exactly numWGs x numCSIters entries."""
+    mix = make_mix(seed=3, n=64)
+    st = storages_for(dev, mix)
+    entries, r = dev.batch(mix, st, reps=1, mode=1, use_graph=1, timeout_s=60.0,
+                           wg_size=256, probe=probe, cohort=cohort)
+    assert r.all_ok == 1
+    check_mix(dev, ohelp, mix, st, entries, 1, 1)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("wg_size", [64, 128, 256])
+def test_batch_cohort_widths_and_modes(dev, ohelp, wg_size, mode):
+    """The cohort with every worker width, in both modes, over a mix with
+    many virtual WGs per worker (blocks of one instance share a worker)."""
+    mix = make_mix(seed=31, n=40, iters_cap=25)
+    st = storages_for(dev, mix)
+    entries, r = dev.batch(mix, st, reps=2, mode=mode, wg_size=wg_size, use_graph=1,
+                           cohort=16, persistent_wgs=128)
+    assert r.all_ok == 1
+    check_mix(dev, ohelp, mix, st, entries, 2, mode)
+
+
 def test_batch_alone_equals_plain(dev):
     mix = make_mix(seed=11, n=8, iters_cap=10)
     for mode in (0, 1):
@@ -112,7 +141,8 @@ def test_batch_alone_equals_plain(dev):
 
 def test_batch_rejects_bad_width_and_mode(dev):
     mix = make_mix(seed=11, n=2, iters_cap=2)
-    for kw in (dict(wg_size=96), dict(wg_size=512), dict(mode=2), dict(probe=2)):
+    for kw in (dict(wg_size=96), dict(wg_size=512), dict(mode=2), dict(probe=2),
+               dict(cohort=-1)):
         with pytest.raises(hs.HSError) as ei:
             dev.batch(mix, storages_for(dev, mix), reps=1, **kw)
         assert ei.value.code == -1  # HS_EINVAL
