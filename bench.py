@@ -269,6 +269,11 @@ def extra_c3(dev, hs, floor_terms):
               for i, (p, nw, it, L) in enumerate(mix)]
     bound_us, crit = max(chains)
     hand = {L: handoff_us(dev, L) for L in sorted({L for *_, L in mix})}
+    # what one-at-a-time hand-offs at the plain form's measured rate would
+    # need for the longest chain (the cohort and the local successor let
+    # the batched run go below it)
+    plain_ref_us, plain_crit = max(
+        (serialized_acquires(p, nw, it, ncu) * hand[L], i) for i, (p, nw, it, L) in enumerate(mix))
     cp, cnw, cit, cL = mix[crit]
     out["critical_path"] = {
         "instance": crit, "prim": hs.prim_name(cp), "num_wgs": cnw, "num_cs_iters": cit,
@@ -280,6 +285,8 @@ def extra_c3(dev, hs, floor_terms):
             rb.roi_ms * 1e3 / serialized_acquires(cp, cnw, cit, ncu), 4),
         "bound_ms": round(bound_us / 1e3, 3),
         "frac_of_bound": round(bound_us / 1e3 / rb.roi_ms, 4),
+        "plain_handoff_chain_ms": round(plain_ref_us / 1e3, 3),
+        "plain_handoff_chain_instance": plain_crit,
         "plain_famutex_us_per_acquire_by_numldst": {
             str(k): round(v, 4) for k, v in hand.items()}}
     return out
