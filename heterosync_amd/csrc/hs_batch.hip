@@ -657,7 +657,8 @@ extern "C" int hs_batch_run(hs_ctx *c, const hs_batch_instance *in, int n,
     if (p == 4 || p == 5) ok = ok && lk[kLkHead] == 0;
     if (p == 7) ok = ok && lk[kLkHead] == (uint32_t)acq && lk[kLkTail] == (uint32_t)acq;
     if (p == 6) ok = ok && lk[kLkTail] == (uint32_t)acq;
-    if (p >= 8) ok = ok && lk[kLkHead] == sem_max_of(p);
+    if (p >= 8)  // an instance nobody entered keeps its initial word
+      ok = ok && lk[kLkHead] == (acq ? sem_max_of(p) : sem_max_of(p) | kSemWaitBit);
     ok = ok && (uint64_t)entries[i] == acq;
   }
   res->all_ok = ok && ctl[0] == 0;

@@ -754,8 +754,10 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
           ok = false;
         }
       }
-      if (is_sem(p) && (!uniq || l < nab))  // count back to max, no writer waiting
-        ok = ok && lk[kLkHead] == sem_max;
+      // count back to max, no writer waiting; a semaphore nobody entered
+      // keeps its initial word (the writer goes first, HSS:23)
+      if (is_sem(p) && (!uniq || l < nab))
+        ok = ok && lk[kLkHead] == (acq ? sem_max : sem_max | kSemWaitBit);
       if (is_prio(p)) ok = ok && lk[kLkPrio] == 0;  // no post in flight
     }
     const uint32_t *lk0 = &locks[0];

@@ -129,6 +129,20 @@ def test_batch_cohort_widths_and_modes(dev, ohelp, wg_size, mode):
     check_mix(dev, ohelp, mix, st, entries, 2, mode)
 
 
+@pytest.mark.parametrize("cohort", [0, 8])
+def test_batch_edge_instances(dev, ohelp, cohort):
+    """Instances at the edges of their arguments in one batch: a single
+    virtual WG, no critical sections at all (storage untouched, no entries),
+    empty critical sections (numLdSt 0), beside ordinary ones."""
+    mix = [(4, 1, 5, 1), (5, 3, 0, 10), (6, 2, 4, 0), (7, 1, 3, 100), (8, 1, 2, 1),
+           (11, 5, 0, 1), (10, 9, 3, 0), (4, 40, 6, 10), (9, 17, 4, 10)]
+    st = storages_for(dev, mix)
+    entries, r = dev.batch(mix, st, reps=2, mode=1, use_graph=1, cohort=cohort)
+    assert r.all_ok == 1
+    check_mix(dev, ohelp, mix, st, entries, 2, 1)
+    assert entries[1] == 0 and entries[5] == 0
+
+
 def test_batch_alone_equals_plain(dev):
     mix = make_mix(seed=11, n=8, iters_cap=10)
     for mode in (0, 1):

@@ -168,6 +168,24 @@ def test_fa_mutex_c1(dev, ohelp):
             assert r.lock_words[0] == r.lock_words[4] == 32000  # ticket == turn
 
 
+@pytest.mark.parametrize("prim", [i for i in range(43) if i != 36])
+def test_edge_shapes(dev, ohelp, prim):
+    """Every primitive id at the edges of its arguments: numLdSt 0 (an empty
+    critical section), numCSIters 0 (no critical section at all, storage
+    untouched), a single WG, and one WG per CU; both modes."""
+    ncu = dev.info.num_cu
+    for L, nw, it in ((0, 1, 1), (1, 1, 0), (0, ncu, 3), (3, 1, 4), (2, 8, 0)):
+        for mode in (0, 1):
+            check(dev, ohelp, prim, L, nw, it, reps=2, mode=mode)
+
+
+@pytest.mark.parametrize("prim", [4, 7, 11, 23, 31])
+def test_long_critical_section(dev, ohelp, prim):
+    """A long critical section (numLdSt 500: each thread's chain is 501
+    lines), in inc mode, where every line of every pass is counted."""
+    check(dev, ohelp, prim, 500, 8, 6, reps=2, mode=1)
+
+
 def test_error_paths(dev):
     ncu = dev.info.num_cu
     with pytest.raises(hs.HSError) as e:
