@@ -27,6 +27,8 @@ def seq_expected(oracle, nw, gs, period, iters, n):
     (1024, 256, 16, 65, 10000),  # relaxed-atomics configuration: 10^4 iterations
     (2048, 64, 8, 33, 100),    # 8 WGs per CU, more writers
     (128, 1024, 2, 16, 20),    # 1024-thread WGs
+    (1, 64, 1, 1, 1),          # one WG: a lone writer, no readers
+    (65, 64, 1, 65, 1),        # one writer, 64 readers, one pass each
 ])
 def test_seqlock(dev, oracle, variant, nw, wg, gs, period, iters):
     seq, out, torn, r = dev.seqlock(nw, wg, gs, period, iters, variant)
@@ -48,6 +50,7 @@ def test_seqlock(dev, oracle, variant, nw, wg, gs, period, iters):
     (1024, 256, 16, 4, 0.33, 200),
     (64, 64, 16, 4, 0.45, 2),       # the next value is read one iteration ahead
     (1024, 256, 16, 4, 0.33, 10000),  # C4
+    (2, 64, 1, 1, 0.33, 1),         # the main WG and a single worker
 ])
 def test_flags(dev, oracle, ntb, tb, accs, reps, cut, iters):
     rnd = np.zeros(ntb * iters, np.float32)
@@ -73,6 +76,7 @@ def test_flags_never_dirty(dev):
     (32, 1024, 256, 100, 8),
     (32, 1024, 256, 10000, 8),   # relaxed-atomics configuration: 10^4 inc/dec pairs
     (4, 512, 64, 50, 0),
+    (1, 1, 64, 1, 0),               # one WG sharing its counters with nobody
 ])
 def test_refcounter(dev, sh, ntb, tb, reps, spacing, strict):
     c0, c1, d0, d1, r = dev.refcounter(sh, ntb, tb, reps, spacing, strict=strict)
@@ -90,6 +94,7 @@ def test_refcounter(dev, sh, ntb, tb, reps, spacing, strict):
     (1024, 256, 10000, 1),    # a WG read after every writer iteration
     (1024, 256, 10000, 2500),  # four reads per WG
     (256, 256, 100, 1000),    # read period past the run: one read at the end
+    (1, 128, 1, 0),           # one WG: one writer wave, one reader wave
 ])
 def test_splitcounter(dev, ntb, tb, iters, read_every):
     counters, out, r = dev.splitcounter(ntb, tb, iters, read_every)
