@@ -16,6 +16,7 @@ namespace {
 
 // accessData (main.hip.cpp:60-72) with the copy replaced by an in-place
 // increment of lines 1..L of the thread's chain; plain loads and stores.
+// tid0: the first thread of the WG-sized region (region index x WG size).
 __device__ inline void cs_inc(int32_t *s, long tid0, int L) {
   const long tid = tid0 + threadIdx.x;
   int32_t *col = s + (tid / 16) * (L + 1) * 16 + threadIdx.x % 16;
@@ -39,8 +40,10 @@ struct Args {
   int L, iters, num_cu, kind;
 };
 
+constexpr int kMaxWg = 256;  // WGs of 64 (the reference's) up to 256 threads
+
 // kind 0 spin, 1 EBO, 3 ticket; every WG works on region 0 (main.hip.cpp:771)
-__global__ void __launch_bounds__(64) k_mutex(Args a) {
+__global__ void __launch_bounds__(kMaxWg) k_mutex(Args a) {
   hs_guard g = hs_guard_begin(a.abort, a.ticks);
   for (int i = 0; i < a.iters; ++i) {
     uint32_t ticket = 0;
@@ -57,8 +60,8 @@ __global__ void __launch_bounds__(64) k_mutex(Args a) {
 
 // kind 0 spin, 1 EBO; one writer WG per group (main.hip.cpp:883), which
 // walks regions 0..perCU-1; readers read region perCU_WGID
-__global__ void __launch_bounds__(64) k_sem(Args a) {
-  __shared__ int32_t sink[64];
+__global__ void __launch_bounds__(kMaxWg) k_sem(Args a) {
+  __shared__ int32_t sink[kMaxWg];
   hs_guard g = hs_guard_begin(a.abort, a.ticks);
   const int b = blockIdx.x, nw = gridDim.x;
   const int nab = nw < a.num_cu ? nw : a.num_cu, per = (nw + nab - 1) / nab;
@@ -70,9 +73,9 @@ __global__ void __launch_bounds__(64) k_sem(Args a) {
                                 : hs_semaphore_ebo_wait(a.state, writer, g);
     if (!ok) return;
     if (writer) {
-      for (int j = 0; j < per; ++j) cs_inc(a.storage, (long)j * 64, a.L);
+      for (int j = 0; j < per; ++j) cs_inc(a.storage, (long)j * blockDim.x, a.L);
     } else {
-      cs_read(a.storage, (long)pw * 64, a.L, sink);
+      cs_read(a.storage, (long)pw * blockDim.x, a.L, sink);
     }
     hs_semaphore_post(a.state, writer);
   }
@@ -80,25 +83,25 @@ __global__ void __launch_bounds__(64) k_sem(Args a) {
 
 // the atomic tree barrier kernel (main.hip.cpp:104-149): own region, barrier,
 // next region
-__global__ void __launch_bounds__(64) k_tree(Args a) {
+__global__ void __launch_bounds__(kMaxWg) k_tree(Args a) {
   hs_guard g = hs_guard_begin(a.abort, a.ticks);
   const hs_tree_barrier bar = hs_tree_barrier_make(a.state, gridDim.x, a.num_cu);
   int cur = blockIdx.x;
   for (int i = 0; i < a.iters; ++i) {
-    cs_inc(a.storage, (long)cur * 64, a.L);
+    cs_inc(a.storage, (long)cur * blockDim.x, a.L);
     if (!hs_tree_barrier_join(bar, g)) return;
     cur = (cur + 1) % gridDim.x;
   }
 }
 
 // the lock-free tree barrier kernel (main.hip.cpp:542-584)
-__global__ void __launch_bounds__(64) k_lf(Args a) {
+__global__ void __launch_bounds__(kMaxWg) k_lf(Args a) {
   hs_guard g = hs_guard_begin(a.abort, a.ticks);
   const hs_lf_barrier bar = hs_lf_barrier_make(a.state, gridDim.x, a.num_cu);
   hs_lf_epoch e = {0, 0};
   int cur = blockIdx.x;
   for (int i = 0; i < a.iters; ++i) {
-    cs_inc(a.storage, (long)cur * 64, a.L);
+    cs_inc(a.storage, (long)cur * blockDim.x, a.L);
     if (!hs_lf_barrier_join(bar, e, g)) return;
     cur = (cur + 1) % gridDim.x;
   }
@@ -113,13 +116,15 @@ __global__ void __launch_bounds__(64) k_lf(Args a) {
 
 // test: 0 mutex (kind 0 spin, 1 EBO, 3 ticket), 1 semaphore (kind 0 spin,
 // 1 EBO; sem_count its size), 2 atomic tree barrier, 3 lock-free barrier.
-// storage: n words, in and out.  Returns 0, -1 bad arguments, -2 HIP error,
+// wg: threads per WG (64 to 256, a multiple of 64).  storage: n words, in
+// and out.  Returns 0, -1 bad arguments, -2 HIP error,
 // -4 the WGs cannot all be resident, -5 the watchdog fired.  *episodes:
 // barrier episodes the device counted.
 extern "C" int hsu_run(int test, int kind, int sem_count, int num_wgs, int num_cu, int L,
-                       int iters, int32_t *storage, long n, double timeout_s,
+                       int iters, int wg, int32_t *storage, long n, double timeout_s,
                        uint32_t *episodes) {
-  if (test < 0 || test > 3 || num_wgs < 1 || num_cu < 1 || L < 0 || iters < 0 || !storage)
+  if (test < 0 || test > 3 || num_wgs < 1 || num_cu < 1 || L < 0 || iters < 0 || !storage ||
+      wg < 64 || wg > kMaxWg || wg % 64)
     return -1;
   const void *kern = test == 0 ? (const void *)k_mutex
                      : test == 1 ? (const void *)k_sem
@@ -127,7 +132,7 @@ extern "C" int hsu_run(int test, int kind, int sem_count, int num_wgs, int num_c
                                  : (const void *)k_lf;
   int dev = 0, per_cu = 0, ncu_dev = 0, khz = 0;
   CK(hipGetDevice(&dev));
-  CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
+  CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, wg, 0));
   CK(hipDeviceGetAttribute(&ncu_dev, hipDeviceAttributeMultiprocessorCount, dev));
   CK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
   if (test > 0 && num_wgs > per_cu * ncu_dev) return -4;
@@ -149,7 +154,7 @@ extern "C" int hsu_run(int test, int kind, int sem_count, int num_wgs, int num_c
   a.iters = iters;
   a.num_cu = num_cu;
   a.kind = kind;
-  hipLaunchKernelGGL((void (*)(Args))kern, dim3(num_wgs), dim3(64), 0, 0, a);
+  hipLaunchKernelGGL((void (*)(Args))kern, dim3(num_wgs), dim3(wg), 0, 0, a);
   CK(hipGetLastError());
   CK(hipDeviceSynchronize());
   uint32_t ab = 0;
