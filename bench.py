@@ -118,8 +118,11 @@ def handoff_floor_us(t, L, lanes=64):
     release is seen by the next holder (one hand-off), which loads the lines
     its predecessor wrote through (a 16-line chunk per round trip, chunks
     pipelined, so at least one load round trip) and drains its stores
-    before releasing (one store round trip)."""
-    return t["hop_other_xcd_us"] + t["load_rt_us"] + t["store_rt_us"]
+    before releasing (one store round trip).  The load is charged at the
+    probe's round trip for lines just written through by agent-scope
+    stores from every XCD (the next holder runs on another XCD 7 times in
+    8, where the lines are not in its L2)."""
+    return t["hop_other_xcd_us"] + t.get("remote_load_rt_us", t["load_rt_us"]) + t["store_rt_us"]
 
 
 # ---- the configurations -----------------------------------------------------
@@ -211,14 +214,19 @@ def batched_acquire_floor_us(p, nw, L, ncu, t, cohort=1):
     its lines (at least one round trip per 64 lines: the widest worker
     splits a chain over four waves, 16 lines per load batch) and drains its
     stores (one store round trip) before releasing.  A semaphore writer
-    walks its group's perCU regions.  With the mutex cohort a spin / EBO
+    walks its group's perCU regions.  The next holder's loads are charged
+    as in handoff_floor_us.  With the mutex cohort a spin / EBO
     lock hold serves up to `cohort` critical sections on one worker, and
     only the first pays the hand-off and the CAS: the bound takes every
     hold at its fullest."""
     regions = 1 if p <= 7 else -(-nw // min(nw, ncu))
     cas = t["rmw_rt_us"] if p in (4, 5) or p >= 8 else 0.0
-    cs = max(1, -(-L // 64)) * t["load_rt_us"] * regions + t["store_rt_us"]
-    handoff = t["hop_other_xcd_us"] + cas + cs
+    rounds = max(1, -(-L // 64)) * regions
+    # the next holder's loads as in C1's floor; a cohort's critical
+    # sections stay on one worker (L2 hits)
+    cold = t.get("remote_load_rt_us", t["load_rt_us"])
+    cs = rounds * t["load_rt_us"] + t["store_rt_us"]
+    handoff = t["hop_other_xcd_us"] + cas + rounds * cold + t["store_rt_us"]
     c = max(1, cohort) if p in (4, 5) else 1
     return (handoff + (c - 1) * cs) / c
 

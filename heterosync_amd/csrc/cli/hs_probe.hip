@@ -224,6 +224,25 @@ __global__ void k_chain(int who, int mode, int n, uint32_t *p, uint32_t *out) {
   out[0] = v;
 }
 
+// Agent-scope stores of the chase's links (chase[line * 32] = next[line])
+// by a grid spread over every XCD: each line is then one another XCD has
+// just written through, 7 times in 8, as a lock's next holder finds the
+// critical section its predecessor wrote.
+__global__ void k_chase_write(const uint32_t *next, uint32_t nl, uint32_t *chase) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x)
+    st_rlx(chase + (size_t)i * 32, next[i]);
+}
+
+// Dependent agent-scope loads of one lane of WG `who` chasing a cyclic
+// permutation of 128-byte lines (chase[line * 32] holds the next line): n
+// steps, each a line no other step of the launch touches.
+__global__ void k_chase(int who, int n, uint32_t start, const uint32_t *chase, uint32_t *out) {
+  if (threadIdx.x != 0 || (int)blockIdx.x != who) return;
+  uint32_t v = start;
+  for (int i = 0; i < n; ++i) v = ld_rlx(chase + (size_t)v * 32);
+  out[0] = v;
+}
+
 // Plain streaming copy of known size: the calibration reference for the
 // FETCH_SIZE / WRITE_SIZE hardware counters.
 __global__ void k_copy(const int4 *__restrict__ src, int4 *__restrict__ dst, long n) {
@@ -281,6 +300,52 @@ int floor_terms(int nwg, int lanes, int L) {
       chain_us[mode] += ms * 1000 / n / 8;
     }
   }
+  // loads of lines another XCD has just written through: a chase over
+  // 64 MiB (512 Ki lines of 128 B, more than the L2s hold) in a random
+  // cyclic order, its links rewritten by agent-scope stores from every XCD
+  // right before each timed launch
+  double remote_us = 0;
+  {
+    const uint32_t nl = 1u << 19;
+    std::vector<uint32_t> order(nl), next(nl);
+    for (uint32_t i = 0; i < nl; ++i) order[i] = i;
+    uint64_t x = 0x9e3779b97f4a7c15ull;
+    for (uint32_t i = nl - 1; i > 0; --i) {  // Fisher-Yates with a xorshift generator
+      x ^= x << 13;
+      x ^= x >> 7;
+      x ^= x << 17;
+      std::swap(order[i], order[x % (i + 1)]);
+    }
+    for (uint32_t i = 0; i < nl; ++i) next[order[i]] = order[(i + 1) % nl];
+    uint32_t *chase, *d_next;
+    CHECK(hipMalloc(&chase, (size_t)nl * 32 * 4));
+    CHECK(hipMalloc(&d_next, (size_t)nl * 4));
+    CHECK(hipMemcpy(d_next, next.data(), (size_t)nl * 4, hipMemcpyHostToDevice));
+    k_chase_write<<<1024, 256>>>(d_next, nl, chase);
+    // each launch starts at its own point of the cycle, 4096 steps apart,
+    // so no launch revisits a line an earlier one loaded
+    const int n = 2000;
+    hipEvent_t e0, e1;
+    CHECK(hipEventCreate(&e0));
+    CHECK(hipEventCreate(&e1));
+    k_chase<<<8, 64>>>(0, n, order[(size_t)16 * 4096], chase, res);  // warm-up launch
+    CHECK(hipDeviceSynchronize());
+    for (int who = 0; who < 8; ++who) {
+      k_chase_write<<<1024, 256>>>(d_next, nl, chase);
+      CHECK(hipDeviceSynchronize());
+      CHECK(hipEventRecord(e0));
+      k_chase<<<8, 64>>>(who, n, order[(size_t)who * 4096], chase, res);
+      CHECK(hipEventRecord(e1));
+      CHECK(hipEventSynchronize(e1));
+      float ms = 0;
+      CHECK(hipEventElapsedTime(&ms, e0, e1));
+      remote_us += ms * 1000 / n / 8;
+    }
+    CHECK(hipEventDestroy(e0));
+    CHECK(hipEventDestroy(e1));
+    CHECK(hipFree(chase));
+    CHECK(hipFree(d_next));
+  }
   auto hop = [&](int a, int b, double *us) {
     const int rounds = 2000;
     CHECK(hipMemset(res, 0, 64));
@@ -310,10 +375,11 @@ int floor_terms(int nwg, int lanes, int L) {
   const double bytes = (double)nwg * lanes * L * 8;  // 4 B load + 4 B store per line
   std::printf("{\"wgs\": %d, \"lanes\": %d, \"num_ldst\": %d, \"l2_pass_us\": %.4f, "
               "\"l2_pass_gbs\": %.1f, \"rmw_rt_us\": %.4f, \"load_rt_us\": %.4f, "
-              "\"store_rt_us\": %.4f, \"hop_same_xcd_us\": %.4f, \"hop_other_xcd_us\": %.4f, "
-              "\"hop_far_xcd_us\": %.4f, \"same_address_rmw_ns\": %.3f}\n",
+              "\"store_rt_us\": %.4f, \"remote_load_rt_us\": %.4f, \"hop_same_xcd_us\": %.4f, "
+              "\"hop_other_xcd_us\": %.4f, \"hop_far_xcd_us\": %.4f, "
+              "\"same_address_rmw_ns\": %.3f}\n",
               nwg, lanes, L, pass_ms * 1000 / passes, bytes * passes / (pass_ms * 1e6),
-              chain_us[0], chain_us[1], chain_us[2], hop_same, hop_other, hop_far,
+              chain_us[0], chain_us[1], chain_us[2], remote_us, hop_same, hop_other, hop_far,
               one_ms * 1e6 / ((double)it * 1024));
   CHECK(hipFree(data));
   CHECK(hipFree(buf));
