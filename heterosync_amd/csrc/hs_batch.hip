@@ -102,8 +102,8 @@ __device__ __forceinline__ void vwg_access(int32_t *s, long tid0, int L) {
 }
 
 // Semaphore reader body (accessData_semRd, main.hip.cpp:89-100) over the
-// same column / slice split: the lane's lines, loaded eight at a time, summed
-// into LDS so the loads are not dead.
+// same column / slice split: the lane's lines, loaded kReadBatch at a time,
+// summed into LDS so the loads are not dead.
 template <int S>
 __device__ __forceinline__ void vwg_read(const int32_t *s, long tid0, int L,
                                          volatile int32_t *sink) {
@@ -112,12 +112,13 @@ __device__ __forceinline__ void vwg_read(const int32_t *s, long tid0, int L,
   const int32_t *c = s + (tid / kWordsPerLine) * (L + 1) * kWordsPerLine + col % kWordsPerLine;
   const int per = (L + S - 1) / S, lo = sl * per, hi = lo + per < L ? lo + per : L;
   int32_t acc = 0;
-  for (int n0 = lo; n0 < hi; n0 += 8) {
-    int32_t v[8];
+  for (int n0 = lo; n0 < hi; n0 += kReadBatch) {
+    int32_t v[kReadBatch];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = n0 + u < hi ? ld_data<true>(c + (n0 + u) * kWordsPerLine) : 0;
+    for (int u = 0; u < kReadBatch; ++u)
+      v[u] = n0 + u < hi ? ld_data<true>(c + (n0 + u) * kWordsPerLine) : 0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc += v[u];
+    for (int u = 0; u < kReadBatch; ++u) acc += v[u];
   }
   sink[threadIdx.x] += acc;
 }

@@ -241,6 +241,9 @@ __device__ __forceinline__ void wg_pass(bool inc_bar, int32_t *s, long tid0, int
   else wg_move<MODE, SC1, 16, false>(s, tid0, L, keep_l2);
 }
 
+// Lines a reader loads before it adds any of them.
+constexpr int kReadBatch = 32;
+
 // Semaphore reader body (accessData_semRd, HM:89-100): L loads, summed
 // into LDS so they are not dead.
 template <bool SC1 = false>
@@ -249,15 +252,17 @@ __device__ __forceinline__ void wg_read(const int32_t *s, long tid0, int L,
   const long tid = tid0 + threadIdx.x;
   const int32_t *col = s + (tid / kWordsPerLine) * (L + 1) * kWordsPerLine +
                        threadIdx.x % kWordsPerLine;
-  // independent loads, issued eight at a time before any is added (a
-  // load-add loop over agent-scope loads waits out one round trip per line)
+  // independent loads, issued 32 at a time before any is added (a load-add
+  // loop over agent-scope loads waits out one round trip per line; eight at
+  // a time waited out one per eight lines)
   int32_t acc = 0;
-  for (int n0 = 0; n0 < L; n0 += 8) {
-    int32_t v[8];
+  for (int n0 = 0; n0 < L; n0 += kReadBatch) {
+    int32_t v[kReadBatch];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = n0 + u < L ? ld_data<SC1>(col + (n0 + u) * kWordsPerLine) : 0;
+    for (int u = 0; u < kReadBatch; ++u)
+      v[u] = n0 + u < L ? ld_data<SC1>(col + (n0 + u) * kWordsPerLine) : 0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc += v[u];
+    for (int u = 0; u < kReadBatch; ++u) acc += v[u];
   }
   sink[threadIdx.x] += acc;
 }
