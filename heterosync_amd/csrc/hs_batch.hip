@@ -71,19 +71,26 @@ __device__ __forceinline__ int find_inst(const BInst *in, int n, int v) {
 // source lines (16 per slice); every load of a band completes before any of
 // its stores (WG barrier), and a band's stores (lines lo+1..hi) never touch
 // the lines the band below still has to read (lines under lo).
-template <int MODE, int S>
+// Loads are agent-scope.  Stores are agent-scope (written through), or with
+// KEEP plain: the lines stay dirty in this XCD's L2, where the next critical
+// section on this worker finds them (agent-scope loads return a line this
+// XCD's L2 holds dirty).  The last section before the instance's state goes
+// to another worker stores at agent scope: an agent-scope store over a dirty
+// line leaves no stale copy behind (the barrier walk's keep_l2 relies on the
+// same two facts).
+template <int MODE, int S, bool KEEP = false>
 __device__ __forceinline__ void vwg_access(int32_t *s, long tid0, int L) {
   const int col = threadIdx.x & 63, sl = threadIdx.x >> 6;
   const long tid = tid0 + col;
   const long base = (tid / kWordsPerLine) * (L + 1);
   const int off = col % kWordsPerLine;
   if (S == 1) {
-    access_data<MODE, true>(s, base, off, L);
+    access_data<MODE, true, !KEEP>(s, base, off, L);
     return;
   }
   if (MODE == 1) {
     const int per = (L + S - 1) / S, lo = sl * per, hi = lo + per < L ? lo + per : L;
-    if (hi > lo) access_data<1, true>(s, base + lo, off, hi - lo);
+    if (hi > lo) access_data<1, true, !KEEP>(s, base + lo, off, hi - lo);
     return;
   }
   int32_t *c = s + base * kWordsPerLine + off;
@@ -96,8 +103,8 @@ __device__ __forceinline__ void vwg_access(int32_t *s, long tid0, int L) {
     else if (n > 0) chunk_tail<0, true, false>(c + (long)lo * kWordsPerLine, n, v);
     wait_mem();
     __syncthreads();
-    if (n == 16) chunk_st<0, 16, true>(c, lo, v);
-    else if (n > 0) chunk_tail<0, true, true>(c + (long)lo * kWordsPerLine, n, v);
+    if (n == 16) chunk_st<0, 16, !KEEP>(c, lo, v);
+    else if (n > 0) chunk_tail<0, !KEEP, true>(c + (long)lo * kWordsPerLine, n, v);
   }
 }
 
@@ -148,6 +155,8 @@ template <int MODE, int S, bool PAR>
 __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
   __shared__ int s_inst[kMaxVpw], s_b[kMaxVpw], s_iter[kMaxVpw], s_phase[kMaxVpw];
   __shared__ uint32_t s_ticket[kMaxVpw], s_skip[kMaxVpw], s_bo[kMaxVpw];
+  __shared__ int s_writer[kMaxVpw];  // mutex WG, or a semaphore's writer (main.hip.cpp:883)
+  __shared__ BInst s_desc[kMaxVpw];  // each slot's instance: no global load on an attempt
   __shared__ int s_remaining, s_enter, s_ok;
   __shared__ unsigned long long s_cand, s_sure;
   __shared__ int32_t sink[64 * S];
@@ -163,10 +172,14 @@ __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
         const int i = find_inst(a.inst, a.ninst, v);
         s_inst[j] = i;
         s_b[j] = v - a.inst[i].vbase;
+        s_desc[j] = a.inst[i];
+        const BInst &I0 = s_desc[j];
+        s_writer[j] = I0.prim < 8 || (s_b[j] / I0.nab) == (s_b[j] % I0.nab) % I0.per;
         s_phase[j] = a.inst[i].iters > 0 ? kIdle : kDone;
         rem += a.inst[i].iters > 0;
       } else {
         s_phase[j] = kDone;
+        s_writer[j] = 0;
       }
       s_iter[j] = 0;
       s_skip[j] = 0;
@@ -254,7 +267,7 @@ __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
       int j = (int)__builtin_ctzll(first ? first : cand);
       cand &= ~(1ull << j);
       if (s_phase[j] == kDone) continue;  // finished by a cohort hand-off this pass
-      const BInst &I = a.inst[s_inst[j]];
+      const BInst &I = s_desc[j];
       const int p = I.prim;
       const bool sem = p >= 8;
       const int b = s_b[j];
@@ -307,58 +320,72 @@ __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
       }
       __syncthreads();
       if (!s_enter) continue;
-      // Virtual WG j holds the lock or has entered the semaphore.  A spin or
-      // EBO mutex may pass straight on to another of this worker's virtual
-      // WGs of the same instance (cohort hand-off): the lock word stays
-      // held, which is the outcome of that WG winning the CAS right after
-      // this one's release; mutual exclusion and the per-WG iteration
-      // counts are unchanged.  A ticket or queue lock releases as always,
-      // and if the next ticket belongs to one of this worker's virtual WGs,
-      // that WG enters at once instead of at the worker's next probe (its
-      // turn has come; the order stays the lock's).
+      // Virtual WG j holds the lock or has entered the semaphore.  Its
+      // successor is chosen before the critical section, when it is one of
+      // this worker's virtual WGs of the same instance (only this worker
+      // changes their state), so that the section can leave its lines in
+      // this XCD's L2 for it:
+      //   * spin / EBO mutex, or a semaphore writer (cohort hand-off, up to
+      //     `cohort` sections per hold): a waiting WG that needs the same
+      //     exclusive hold takes it over without the release and the next
+      //     attempt -- the outcome of that WG winning right after the
+      //     release; mutual exclusion and the per-WG iteration counts are
+      //     unchanged;
+      //   * ticket / queue lock: the lock is released as always, and the
+      //     owner of the next ticket, if it is here, enters at once instead
+      //     of at the worker's next probe (its turn has come; the order
+      //     stays the lock's).
+      uint32_t held = 0;  // exclusive entries of this hold, counted at its end
+      // the chain stays within one instance: its descriptor is read once
+      const BInst H = s_desc[j];
       for (int chain = 1;; ++chain) {
-        const BInst &H = a.inst[s_inst[j]];
         const int hp = H.prim;
         const bool hsem = hp >= 8;
         const int hb = s_b[j];
-        const int hgrp = hb % H.nab, hpw = hb / H.nab;
-        const bool hwriter = !hsem || hpw == hgrp % H.per;  // main.hip.cpp:883
-        // ---- critical section of virtual WG hb of instance H ------------
-        if (!hsem) {
-          vwg_access<MODE, S>(H.storage, 0, H.L);
-        } else if (hwriter) {
-          for (int k = 0; k < H.per; ++k) vwg_access<MODE, S>(H.storage, (long)k * 64, H.L);
-        } else {
-          vwg_read<S>(H.storage, (long)hpw * 64, H.L, sink);
-        }
-        wait_mem();  // every lane's CS accesses complete before the release
-        __syncthreads();
+        const int hpw = hb / H.nab;
+        const bool hwriter = s_writer[j] != 0;
+        const bool fifo = hp == 6 || hp == 7;
+        const bool excl = hp == 4 || hp == 5 || (hsem && hwriter);
         int nxt = -1;
-        if ((hp == 4 || hp == 5) && chain < a.cohort) {
+        if ((excl && chain < a.cohort) || fifo) {
           if (lane < 64) {
-            const bool want = lane < a.vpw && lane != j && s_phase[lane] != kDone &&
-                              s_inst[lane] == s_inst[j];
+            bool want = lane < a.vpw && lane != j && s_inst[lane] == s_inst[j];
+            want = want && (fifo ? s_phase[lane] == kWaiting && s_ticket[lane] == s_ticket[j] + 1
+                                 : s_phase[lane] != kDone && s_writer[lane] != 0);
             const unsigned long long m = __ballot(want);
             if (lane == 0) {
-              // the next waiting virtual WG after j, cyclically
+              // the next one after j, cyclically
               const unsigned long long hi = j < 63 ? m & (~0ull << (j + 1)) : 0;
               s_enter = m == 0 ? -1 : (int)__builtin_ctzll(hi ? hi : m);
             }
           }
           __syncthreads();
           nxt = s_enter;
-        } else if (hp == 6 || hp == 7) {
-          if (lane < 64) {
-            const bool next = lane < a.vpw && lane != j && s_phase[lane] == kWaiting &&
-                              s_inst[lane] == s_inst[j] && s_ticket[lane] == s_ticket[j] + 1;
-            const unsigned long long m = __ballot(next);
-            if (lane == 0) s_enter = m == 0 ? -1 : (int)__builtin_ctzll(m);
-          }
-          __syncthreads();
-          nxt = s_enter;
         }
+        // ---- critical section of virtual WG hb of instance H ------------
+        if (!hsem) {
+          if (nxt >= 0) vwg_access<MODE, S, true>(H.storage, 0, H.L);
+          else vwg_access<MODE, S, false>(H.storage, 0, H.L);
+        } else if (hwriter) {
+          for (int k = 0; k < H.per; ++k) {
+            if (nxt >= 0) vwg_access<MODE, S, true>(H.storage, (long)k * 64, H.L);
+            else vwg_access<MODE, S, false>(H.storage, (long)k * 64, H.L);
+          }
+        } else {
+          vwg_read<S>(H.storage, (long)hpw * 64, H.L, sink);
+        }
+        // Every lane's CS accesses complete before the release.  Before a
+        // section that stays on this worker the drain is only needed where
+        // another wave reads the lines next (copy mode's bands over several
+        // waves); elsewhere each word is next touched by the same lane,
+        // after its own store.
+        if (nxt < 0 || fifo || (MODE == 0 && S > 1)) wait_mem();
+        __syncthreads();
         if (lane == 0) {
           uint32_t *lk = H.lk;
+          // entry counts, off the hand-off path: one add per hold (a
+          // returning RMW in every section would make the next section's
+          // drain wait out its round trip)
           if (!hsem) {
             if (hp == 4 || hp == 5) {
               if (nxt < 0) mutex_unlock<kSpin, true>(lk, nullptr, 0, 0);
@@ -367,10 +394,17 @@ __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
             } else {
               mutex_unlock<kSleep, true>(lk, H.ring, H.ring_stride, s_ticket[j]);
             }
-            add_rlx(lk + kLkCs, 1);  // entry count, off the hand-off path
+            ++held;
+            if (nxt < 0) count_rlx_n(lk + kLkCs, held);
+          } else if (hwriter) {
+            ++held;
+            if (nxt < 0) {
+              sem_post<false, true>(lk, H.sem_max);
+              count_rlx_n(lk + kLkCs, held);
+            }
           } else {
-            sem_post<false, true>(lk, hwriter ? H.sem_max : 1u);
-            add_rlx(lk + (hwriter ? kLkCs : kLkRd), 1);
+            sem_post<false, true>(lk, 1u);
+            count_rlx_n(lk + kLkRd, 1);
           }
           s_bo[j] = a.bo.min_q;
           s_skip[j] = 0;
@@ -505,8 +539,10 @@ BatchFn pick_batch(const hs_batch_cfg *cfg) {
 // run, and in blocks a spin / EBO lock holder finds the instance's other
 // waiters on its own worker, and a worker's WGs take consecutive tickets of
 // a ticket or queue lock (one wave-wide fetch-add), so the next ticket's
-// owner is usually local.  Semaphores stay round robin over the free
-// slots: their readers run concurrently only on different workers.
+// owner is usually local.  Semaphores whose readers enter one or two at a
+// time go in blocks as well (their writers hand over in cohorts); the wider
+// ones stay round robin over the free slots, since their readers run
+// concurrently only on different workers.
 std::vector<int> place_vwgs(const Batch &B, int P, int vpw, bool blocks) {
   std::vector<int> map((size_t)P * vpw, -1);
   if (!blocks) {
@@ -515,8 +551,12 @@ std::vector<int> place_vwgs(const Batch &B, int P, int vpw, bool blocks) {
   }
   std::vector<int> fill(P, 0);
   int w = 0;
+  // in blocks: every mutex, and the semaphores whose readers enter one or
+  // two at a time (spinSem1/2, spinSemEBO1/2), whose writers hand over in
+  // cohorts like the mutexes
+  auto blocked = [](int p) { return p <= 7 || ((p - 8) % 4) < 2; };
   for (const BInst &I : B.h) {
-    if (I.prim > 7) continue;  // semaphores below
+    if (!blocked(I.prim)) continue;
     for (int k = 0; k < I.nw; ++k) {
       while (w < P && fill[w] == vpw) ++w;
       if (w == P) return {};
@@ -525,7 +565,7 @@ std::vector<int> place_vwgs(const Batch &B, int P, int vpw, bool blocks) {
   }
   int rr = 0;
   for (const BInst &I : B.h) {
-    if (I.prim <= 7) continue;
+    if (blocked(I.prim)) continue;
     for (int k = 0; k < I.nw; ++k) {
       for (int tries = 0; fill[rr] == vpw; ++tries) {
         if (tries == P) return {};

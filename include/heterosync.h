@@ -298,14 +298,16 @@ typedef struct {
                         turn (one round trip each), 1 = one load per lane
                         probes all of a worker's virtual WGs at once, then
                         lane 0 acts on the hopeful ones */
-  int cohort;        /* spin and EBO mutexes: most critical sections per
-                        hold of the lock word.  A worker that holds an
-                        instance's lock passes it to another of its virtual
-                        WGs of that instance still waiting, without the
-                        release store and the next CAS (the same outcome as
-                        that WG winning the CAS right after a release), up to
+  int cohort;        /* spin and EBO mutexes and semaphore writers: most
+                        critical sections per exclusive hold.  A worker that
+                        holds an instance's lock (or a semaphore's whole
+                        count) passes it to another of its virtual WGs of
+                        that instance still waiting for it, without the
+                        release and the next attempt (the same outcome as
+                        that WG winning right after a release), up to
                         `cohort` critical sections, then releases.  Every
-                        mutex instance's virtual WGs are then placed in
+                        mutex instance's virtual WGs, and those of the
+                        semaphores of count 1 and 2, are then placed in
                         blocks on workers instead of round robin (a ticket
                         or queue lock's next ticket is then usually on the
                         releasing worker, whose WG enters at once).  0 or

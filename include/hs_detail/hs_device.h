@@ -53,6 +53,9 @@ __device__ __forceinline__ uint32_t inc_wrap_rlx(uint32_t *p, uint32_t limit) {
 __device__ __forceinline__ void count_rlx(uint32_t *p) {
   (void)__hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, HS_AGENT);
 }
+__device__ __forceinline__ void count_rlx_n(uint32_t *p, uint32_t n) {
+  (void)__hip_atomic_fetch_add(p, n, __ATOMIC_RELAXED, HS_AGENT);
+}
 __device__ __forceinline__ uint32_t add_rel(uint32_t *p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELEASE, HS_AGENT);
 }

@@ -109,7 +109,9 @@ __device__ __forceinline__ void chunk_tail(int32_t *col, int r, int32_t (&v)[16]
 // complete in order).  The memory counter counts loads and stores in issue
 // order, so waiting for a chunk's loads never waits for stores issued after
 // them: each chunk costs one load round trip instead of a load and a store.
-template <int MODE, bool SC1 = false>
+// SC1: agent-scope loads and stores; SC1_ST (default SC1) the stores' scope
+// alone (SC1 loads with plain stores leave the lines dirty in this XCD's L2).
+template <int MODE, bool SC1 = false, bool SC1_ST = SC1>
 __device__ __forceinline__ void access_data(int32_t *__restrict__ s,
                                             long base_line, int off, int L) {
   if (L <= 0) return;
@@ -118,21 +120,21 @@ __device__ __forceinline__ void access_data(int32_t *__restrict__ s,
   int32_t a[16], b[16];
   if (full == 0) {
     chunk_tail<MODE, SC1, false>(col, r, a);
-    chunk_tail<MODE, SC1, true>(col, r, a);
+    chunk_tail<MODE, SC1_ST, true>(col, r, a);
     return;
   }
   int lo = L - 16;
   chunk_ld<MODE, 16, SC1>(col, lo, a);
   for (int f = 1; f < full; ++f) {
     chunk_ld<MODE, 16, SC1>(col, lo - 16, b);
-    chunk_st<MODE, 16, SC1>(col, lo, a);
+    chunk_st<MODE, 16, SC1_ST>(col, lo, a);
     lo -= 16;
 #pragma unroll
     for (int k = 0; k < 16; ++k) a[k] = b[k];
   }
   if (r > 0) chunk_tail<MODE, SC1, false>(col, r, b);
-  chunk_st<MODE, 16, SC1>(col, lo, a);
-  if (r > 0) chunk_tail<MODE, SC1, true>(col, r, b);
+  chunk_st<MODE, 16, SC1_ST>(col, lo, a);
+  if (r > 0) chunk_tail<MODE, SC1_ST, true>(col, r, b);
 }
 
 // Lane `t` of a WG whose data region starts at global thread id `tid0`.
