@@ -157,7 +157,8 @@ __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
   __shared__ uint32_t s_ticket[kMaxVpw], s_skip[kMaxVpw], s_bo[kMaxVpw];
   __shared__ int s_writer[kMaxVpw];  // mutex WG, or a semaphore's writer (main.hip.cpp:883)
   __shared__ BInst s_desc[kMaxVpw];  // each slot's instance: no global load on an attempt
-  __shared__ int s_remaining, s_enter, s_ok;
+  __shared__ int s_remaining, s_enter, s_ok, s_hold_n;
+  __shared__ unsigned long long s_hold_m;
   __shared__ unsigned long long s_cand, s_sure;
   __shared__ int32_t sink[64 * S];
   const int lane = threadIdx.x, w = blockIdx.x;
@@ -338,6 +339,69 @@ __global__ void __launch_bounds__(64 * S) k_batch(BArgs a) {
       uint32_t held = 0;  // exclusive entries of this hold, counted at its end
       // the chain stays within one instance: its descriptor is read once
       const BInst H = s_desc[j];
+      if (a.cohort > 1 && (H.prim == 4 || H.prim == 5 || (H.prim >= 8 && s_writer[j] != 0))) {
+        // Cohort hold.  Every critical section of a mutex instance, and
+        // every writer section of a semaphore, walks the same lines, so the
+        // hold runs its sections back to back: as many as this worker's
+        // waiting WGs of the instance still owe, up to `cohort`, each a full
+        // walk of the lines (loads and stores), all but the last leaving
+        // them in this XCD's L2.  A lane touches the same words in every
+        // section (copy mode over several waves excepted, which drains and
+        // synchronizes between sections), so no barrier separates them.
+        // The sections are then credited to those WGs round robin from j,
+        // and the lock (or the semaphore's count) is released once.
+        if (lane < 64) {
+          const bool want = lane < a.vpw && s_phase[lane] != kDone &&
+                            s_inst[lane] == s_inst[j] && s_writer[lane] != 0;
+          const unsigned long long m = __ballot(want);
+          if (lane == 0) {
+            uint32_t owed = 0;
+            for (unsigned long long r = m; r; r &= r - 1)
+              owed += (uint32_t)(H.iters - s_iter[__builtin_ctzll(r)]);
+            s_hold_n = (int)(owed < (uint32_t)a.cohort ? owed : (uint32_t)a.cohort);
+            s_hold_m = m;
+          }
+        }
+        __syncthreads();
+        const int n = s_hold_n;
+        const int nreg = H.prim >= 8 ? H.per : 1;  // a semaphore writer walks perCU regions
+        for (int c = 0; c < n; ++c) {
+          for (int k = 0; k < nreg; ++k) {
+            if (c + 1 < n) vwg_access<MODE, S, true>(H.storage, (long)k * 64, H.L);
+            else vwg_access<MODE, S, false>(H.storage, (long)k * 64, H.L);
+          }
+          if (MODE == 0 && S > 1) {
+            wait_mem();
+            __syncthreads();
+          }
+        }
+        wait_mem();  // every lane's accesses complete before the release
+        __syncthreads();
+        if (lane == 0) {
+          unsigned long long m = s_hold_m;
+          int k = j;
+          for (int c = 0; c < n; ++c) {
+            s_bo[k] = a.bo.min_q;
+            s_skip[k] = 0;
+            if (++s_iter[k] == H.iters) {
+              s_phase[k] = kDone;
+              --s_remaining;
+              m &= ~(1ull << k);
+            } else {
+              s_phase[k] = kIdle;
+            }
+            if (!m) break;
+            const unsigned long long hi = k < 63 ? m & (~0ull << (k + 1)) : 0;
+            k = (int)__builtin_ctzll(hi ? hi : m);  // the next waiting WG, cyclically
+          }
+          if (H.prim < 8) mutex_unlock<kSpin, true>(H.lk, nullptr, 0, 0);
+          else sem_post<false, true>(H.lk, H.sem_max);
+          count_rlx_n(H.lk + kLkCs, (uint32_t)n);
+        }
+        progress = 1;
+        __syncthreads();
+        continue;
+      }
       for (int chain = 1;; ++chain) {
         const int hp = H.prim;
         const bool hsem = hp >= 8;
