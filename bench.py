@@ -215,19 +215,20 @@ def batched_acquire_floor_us(p, nw, L, ncu, t, cohort=1):
     splits a chain over four waves, 16 lines per load batch) and drains its
     stores (one store round trip) before releasing.  A semaphore writer
     walks its group's perCU regions.  The next holder's loads are charged
-    as in handoff_floor_us.  With the mutex cohort a spin / EBO
-    lock hold serves up to `cohort` critical sections on one worker, and
-    only the first pays the hand-off and the CAS: the bound takes every
-    hold at its fullest."""
+    as in handoff_floor_us.  With the cohort a spin / EBO lock or a
+    semaphore writer's hold serves up to `cohort` critical sections on one
+    worker, back to back: only the first pays the hand-off and the CAS, and
+    each section at least one load round trip through the L2 (no drain
+    between them); the bound takes every hold at its fullest."""
     regions = 1 if p <= 7 else -(-nw // min(nw, ncu))
     cas = t["rmw_rt_us"] if p in (4, 5) or p >= 8 else 0.0
     rounds = max(1, -(-L // 64)) * regions
     # the next holder's loads as in C1's floor; a cohort's critical
     # sections stay on one worker (L2 hits)
     cold = t.get("remote_load_rt_us", t["load_rt_us"])
-    cs = rounds * t["load_rt_us"] + t["store_rt_us"]
+    cs = rounds * t["load_rt_us"]
     handoff = t["hop_other_xcd_us"] + cas + rounds * cold + t["store_rt_us"]
-    c = max(1, cohort) if p in (4, 5) else 1
+    c = max(1, cohort) if p in (4, 5) or p >= 8 else 1
     return (handoff + (c - 1) * cs) / c
 
 
