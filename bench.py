@@ -91,26 +91,68 @@ def probe_floor(nw, wg, L):
     return json.loads(p.stdout.strip().splitlines()[-1])
 
 
-def barrier_floor_us(name, t):
-    """Lower bound on one episode of a grid barrier: the data pass (the
-    local-exchange forms move two regions per episode) plus the dependent
-    steps no episode can overlap with its data.  The XCD-aware atomic tree:
-    two RMW round trips (sub-counter, device counter), then a release that
-    must reach every XCD (the slowest cross-XCD hop).  The flag tree: the
-    member flag to its sub-group leader (same XCD), the leaders' flags to
-    the device master (which must see the farthest leader's) and the
-    release (which must reach the farthest XCD).  The local exchange adds
-    one same-XCD hop (the group's all-to-all flags or counter)."""
+def lf_xcd_form(nw, wg, ncu=256, nxcd=8):
+    """Whether the XCD-aware lock-free barrier runs its XCD form (one
+    sub-group per XCD, the XCD leaders all-to-all, each releasing its own
+    XCD): the library takes it when the XCD leader has a lane for each of
+    its XCD's other WGs (lf_xcd_form in hs_syncprims.hip)."""
+    nab = min(nw, ncu)
+    per = -(-nw // nab)
+    return (-(-nab // nxcd)) * per - 1 <= wg
+
+
+def barrier_floor_us(name, t, ncu=256):
+    """Lower bound on one episode of a grid barrier at the probed shape
+    (t["wgs"] x t["lanes"]): the data pass (the local-exchange forms move
+    two regions per episode) plus the dependent steps no episode can overlap
+    with its data.  The XCD-aware atomic tree: two RMW round trips
+    (sub-counter, device counter), then a release that must reach every XCD
+    (the slowest cross-XCD hop).  The flag tree: the member flags reach
+    their leader inside the XCD (plain stores kept in the XCD's L2:
+    hop_same_xcd_l2), then, in the XCD form, the XCD leaders' flags reach
+    every other leader (the slowest cross-XCD hop) and each leader releases
+    its own XCD (same-XCD hop again); in the device form the leaders' flags
+    reach the device master and its release reaches every XCD (two
+    cross-XCD hops).  The local exchange adds one same-XCD hop (the group's
+    all-to-all flags, or its counter and generation)."""
     lx = "LocalExch" in name
     data = t["l2_pass_us"] * (2 if lx else 1)
     far = t.get("hop_far_xcd_us", t["hop_other_xcd_us"])
+    same_l2 = t.get("hop_same_xcd_l2_us", t["hop_same_xcd_us"])
     if name.startswith("atomic"):
         sync = 2 * t["rmw_rt_us"] + far
+        if lx:
+            sync += t["hop_same_xcd_us"]
     else:
-        sync = t["hop_same_xcd_us"] + 2 * far
-    if lx:
-        sync += t["hop_same_xcd_us"]
+        if lf_xcd_form(t["wgs"], t["lanes"], ncu):
+            sync = 2 * same_l2 + far
+        else:
+            sync = same_l2 + 2 * far
+        if lx:
+            sync += same_l2
     return data + sync
+
+
+def floor_text(name, nw, wg, ncu):
+    """What barrier_floor_us charges for `name`, in words."""
+    data = ("one episode's data pass through the L2s at this footprint (hs_probe floor, the "
+            "kernel's own walk with no barrier)" + (", twice (two regions)" if "LocalExch" in name
+                                                   else ""))
+    if name.startswith("atomic"):
+        sync = ("2 dependent RMW round trips (mean over the 8 issuing XCDs) + the release "
+                "hand-off to the farthest XCD (slowest of the 7 cross-XCD hops)")
+        if "LocalExch" in name:
+            sync += " + one same-XCD hop (the group's counter and generation)"
+    elif lf_xcd_form(nw, wg, ncu):
+        sync = ("the member flags' hop to their XCD leader (plain stores kept in the XCD's L2) "
+                "+ the XCD leaders' all-to-all (slowest cross-XCD hop) + each leader's "
+                "release of its own XCD (same-XCD hop)")
+    else:
+        sync = ("the member flags' hop to their leader (same XCD) + the leaders' flags to the "
+                "device master and its release to every XCD (two slowest cross-XCD hops)")
+    if not name.startswith("atomic") and "LocalExch" in name:
+        sync += " + the group's all-to-all flags (same-XCD hop)"
+    return data + " + " + sync
 
 
 def handoff_floor_us(t, L, lanes=64):
@@ -167,7 +209,7 @@ def extra_barriers(dev, hs, a, nw, floor_terms):
                     "episode_us": round(ep_us, 3),
                     "parity_ok": bool(np.array_equal(s, want)) and r.state_ok == 1}
             if floor_terms and not r.sync_scope_used:
-                fl = barrier_floor_us(name, floor_terms)
+                fl = barrier_floor_us(name, floor_terms, dev.info.num_cu)
                 line.update(floor_us=round(fl, 3), frac_of_floor=round(fl / ep_us, 4))
             out[key] = line
     return out
@@ -565,7 +607,7 @@ def main():
     value = episodes / (roi_ms / 1e3)
     avg_launch_ms = kern_ms / (reps * a.steps)
     ep_us = avg_launch_ms * 1e3 / a.iters
-    floor_us = barrier_floor_us(a.prim, floor_terms) if floor_terms else None
+    floor_us = barrier_floor_us(a.prim, floor_terms, ncu) if floor_terms else None
     # HBM view: algorithmic bytes per launch = episodes/launch x WGs x lanes x
     # numLdSt x (4 B load + 4 B store), over the average launch duration
     bytes_per_episode = nw * a.wg * a.num_ldst * 8
@@ -605,10 +647,7 @@ def main():
             "avg_launch_ms": round(avg_launch_ms, 4),
             "episode_us": round(ep_us, 4),
             "floor_us": round(floor_us, 4) if floor_us else None,
-            "floor": ("one episode's data pass through the L2s at this footprint "
-                      "(hs_probe floor, the kernel's own walk with no barrier) + 2 dependent "
-                      "RMW round trips (mean over the 8 issuing XCDs) + the release "
-                      "hand-off to the farthest XCD (slowest of the 7 cross-XCD hops)"),
+            "floor": floor_text(a.prim, nw, a.wg, ncu),
             "floor_terms": floor_terms,
             "hbm": {"achieved_gbs": round(achieved_gbs, 1), "peak_gbs": HBM_PEAK_GBS,
                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),

@@ -58,7 +58,13 @@ __global__ void k_fence(int mode, int iters, int32_t *data, uint32_t *sink) {
   if (threadIdx.x == 0) sink[blockIdx.x] = mine[0];
 }
 
-__global__ void k_pingpong(int a, int b, int rounds, uint32_t *flag, uint32_t *res) {
+// One-way hand-off between WGs a and b: ping-pong on one flag, polled with
+// agent-scope loads.  plain = 0: agent-scope stores (written through);
+// plain = 1: plain stores, which stay dirty in the storing XCD's L2 (only
+// for two WGs on one XCD: the form the barriers use for flags that never
+// leave an XCD).
+__global__ void k_pingpong(int a, int b, int rounds, uint32_t *flag, uint32_t *res,
+                           int plain) {
   if (threadIdx.x != 0) return;
   const int me = blockIdx.x;
   if (me != a && me != b) return;
@@ -71,7 +77,8 @@ __global__ void k_pingpong(int a, int b, int rounds, uint32_t *flag, uint32_t *r
         res[2] = 1;
         return;
       }
-    st_rlx(flag, want + 1);
+    if (plain) st_xcd(flag, want + 1);
+    else st_rlx(flag, want + 1);
   }
 }
 
@@ -346,12 +353,12 @@ int floor_terms(int nwg, int lanes, int L) {
     CHECK(hipFree(chase));
     CHECK(hipFree(d_next));
   }
-  auto hop = [&](int a, int b, double *us) {
+  auto hop = [&](int a, int b, double *us, int plain) {
     const int rounds = 2000;
     CHECK(hipMemset(res, 0, 64));
     const float ms = time_ms([&] {
       CHECK(hipMemset(buf, 0, 256));
-      k_pingpong<<<16, 64>>>(a, b, rounds, buf, res);
+      k_pingpong<<<16, 64>>>(a, b, rounds, buf, res, plain);
     });
     uint32_t hr[4];
     CHECK(hipMemcpy(hr, res, 16, hipMemcpyDeviceToHost));
@@ -359,13 +366,15 @@ int floor_terms(int nwg, int lanes, int L) {
     *us = ms * 1000 / (2 * rounds);
     return true;
   };
-  double hop_same = 0, hop_other = 0, hop_far = 0, us = 0;
+  double hop_same = 0, hop_same_l2 = 0, hop_other = 0, hop_far = 0, us = 0;
   for (int j = 0; j < 8; ++j) {  // WG j and WG j + 8 share an XCD
-    if (!hop(j, j + 8, &us)) return 2;
+    if (!hop(j, j + 8, &us, 0)) return 2;
     hop_same += us / 8;
+    if (!hop(j, j + 8, &us, 1)) return 2;
+    hop_same_l2 += us / 8;
   }
   for (int j = 1; j < 8; ++j) {  // WG 0 and WG j do not
-    if (!hop(0, j, &us)) return 2;
+    if (!hop(0, j, &us, 0)) return 2;
     hop_other += us / 7;
     hop_far = std::max(hop_far, us);
   }
@@ -376,10 +385,12 @@ int floor_terms(int nwg, int lanes, int L) {
   std::printf("{\"wgs\": %d, \"lanes\": %d, \"num_ldst\": %d, \"l2_pass_us\": %.4f, "
               "\"l2_pass_gbs\": %.1f, \"rmw_rt_us\": %.4f, \"load_rt_us\": %.4f, "
               "\"store_rt_us\": %.4f, \"remote_load_rt_us\": %.4f, \"hop_same_xcd_us\": %.4f, "
+              "\"hop_same_xcd_l2_us\": %.4f, "
               "\"hop_other_xcd_us\": %.4f, \"hop_far_xcd_us\": %.4f, "
               "\"same_address_rmw_ns\": %.3f}\n",
               nwg, lanes, L, pass_ms * 1000 / passes, bytes * passes / (pass_ms * 1e6),
-              chain_us[0], chain_us[1], chain_us[2], remote_us, hop_same, hop_other, hop_far,
+              chain_us[0], chain_us[1], chain_us[2], remote_us, hop_same, hop_same_l2, hop_other,
+              hop_far,
               one_ms * 1e6 / ((double)it * 1024));
   CHECK(hipFree(data));
   CHECK(hipFree(buf));
@@ -434,7 +445,7 @@ int main(int argc, char **argv) {
     CHECK(hipMemset(res, 0, 64));
     const float ms = time_ms([&] {
       CHECK(hipMemset(buf, 0, 256));
-      k_pingpong<<<16, 64>>>(pr.first, pr.second, rounds, buf, res);
+      k_pingpong<<<16, 64>>>(pr.first, pr.second, rounds, buf, res, 0);
     });
     uint32_t hr[4];
     CHECK(hipMemcpy(hr, res, 16, hipMemcpyDeviceToHost));

@@ -355,6 +355,19 @@ int sub_group_size(int grid, bool flags) {
   return s;
 }
 
+// Whether the XCD-aware lock-free barrier takes its XCD form (one
+// sub-group per XCD, the XCD leaders all-to-all, each releasing its own
+// XCD): when the leader has a lane for each of its XCD's other WGs, so its
+// gather polls one flag per lane.  Measured per episode (numLdSt 10, two
+// forms: device / XCD): 256 x 64 3.54 / 3.52 us, 512 x 256 3.76 / 3.60,
+// 1024 x 256 4.16 / 4.03, 512 x 512 4.41 / 4.17, 256 x 1024 4.79 / 4.63;
+// with several flags per lane the device form wins (1024 x 64 3.87 / 3.88,
+// 2048 x 64 4.06 / 4.39, 4096 x 64 5.53 / 6.69).
+bool lf_xcd_form(int nab, int per, int nxcd, int wg) {
+  const int nwx = (nab / nxcd + (nab % nxcd ? 1 : 0)) * per;  // most WGs on one XCD
+  return nwx - 1 <= wg;
+}
+
 bool is_barrier(int p) { return p < 4 || (p >= 37 && p <= 42); }
 bool is_mutex(int p) { return (p >= 4 && p <= 7) || (p >= 20 && p <= 23); }
 bool is_sem(int p) { return (p >= 8 && p <= 19) || (p >= 24 && p <= 35); }
@@ -582,7 +595,9 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   // barrier words, kBarWords blocks: [global][XCD-aware sub-counters:
   // nxcd x sub_max][groups: nab][release words: nxcd][episode count line]
   int sub_max = 0, n_sub = 0;  // XCD-aware tree: sub-counters per XCD, in all
-  const int sub = sub_group_size(nw, p == 2 || p == 3);
+  const bool lf = p == 2 || p == 3;
+  const bool a2a = xcd_mode && lf && lf_xcd_form(nab, per, nxcd, wg);
+  const int sub = a2a ? (nab / nxcd + (nab % nxcd ? 1 : 0)) * per : sub_group_size(nw, lf);
   if (xcd_mode)
     for (int x = 0; x < nxcd; ++x) {
       const int nwx = (nab / nxcd + (x < nab % nxcd ? 1 : 0)) * per;  // WGs on XCD x
@@ -647,7 +662,7 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
     const size_t S = kFlagStride, o_din = nxcd, o_sin = o_din + n_sub_slots;
     const size_t o_lin = o_sin + n_sub_slots * sub, o_lout = o_lin + (size_t)nab * per;
     a.xf = XcdFlags{fl, fl + o_din * S, fl + o_sin * S, fl + o_lin * S, fl + o_lout * S,
-                    episodes, nab, per, nxcd, sub_max, sub};
+                    episodes, nab, per, nxcd, sub_max, sub, a2a ? 1 : 0};
   }
   a.flat = d_flat;
   a.nxcd = nxcd;

@@ -39,6 +39,15 @@ __device__ __forceinline__ void st_rlx(uint32_t *p, uint32_t v) {
 __device__ __forceinline__ void st_rel(uint32_t *p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, HS_AGENT);
 }
+// A flag store that stays in this XCD's L2: a plain store (no cache-policy
+// bits), left dirty in the L2 of the storing CU's XCD, where agent-scope
+// loads from the same XCD return it without a memory round trip.  Only for
+// words that, within one launch, are stored and loaded by WGs of one XCD
+// alone: a word also stored by another XCD could leave a stale dirty copy
+// here.
+__device__ __forceinline__ void st_xcd(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
 __device__ __forceinline__ uint32_t add_rlx(uint32_t *p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, HS_AGENT);
 }

@@ -694,7 +694,8 @@ __device__ __forceinline__ bool xcd_tree_arrive(const XcdTree &t, const XcdSeat 
     if (per <= 1) return true;
     uint32_t *cnt = t.grp + (long)grp * kBarWords, *gen = cnt + 32;
     if (inc_wrap_rlx(cnt, per - 1) != per - 1) return poll_eq_pipe(gen, epoch, poll_q, g);
-    st_rlx(gen, epoch);
+    // a group lies inside one XCD: its generation stays in that XCD's L2
+    st_xcd(gen, epoch);
     return true;
   }
   uint32_t *rel = seat.rel;
@@ -723,18 +724,41 @@ struct XcdFlags {
   int nab, per_cu, nxcd;
   int sub_max;           // sub-groups per XCD (at most), as for XcdTree
   int sub_wgs;           // WGs per sub-group
+  int a2a;               // XCD form: one sub-group per XCD, leaders all-to-all
 };
+
+// Poll until the word has reached `v` (wrapping compare), for a flag that
+// its writer can move one epoch past `v` before this poll has seen `v`.
+__device__ __forceinline__ bool poll_ge(const uint32_t *p, uint32_t v,
+                                        uint32_t poll_q, Guard &g) {
+  while ((int32_t)(ld_rlx(p) - v) < 0) {
+    if (!g.ok()) return false;
+    sleep_quanta(poll_q);
+  }
+  return true;
+}
 
 // Whole-WG call.  el = group-level epoch, eg = episode (device level).
 // The global barrier gathers arrival flags on two levels, as the tree
 // barrier counts them: the WGs of one XCD form sub-groups of up to
-// sub_wgs (numbered as in xcd_tree_arrive), each leader gathers its
-// members' flags one per lane, and the device master (leader of sub-group 0)
-// gathers the leaders' flags one per lane.  The release is single-hop: the
-// device master writes one release word per XCD (gout[x]) and every WG of
-// that XCD polls its word.  The local-exchange barrier is all-to-all: each
-// member stores its in-flag and polls the other members' in-flags.  Flags and data are
-// agent-scope accesses, so no level needs a cache fence.
+// sub_wgs (numbered as in xcd_tree_arrive), and each leader gathers its
+// members' flags one per lane.  Then one of two forms (the host picks):
+//   * XCD form (a2a; one sub-group per XCD, when the leader has a lane for
+//     each of its XCD's other WGs): the 8 XCD leaders meet all-to-all --
+//     each publishes its XCD's arrival and gathers the other leaders' --
+//     and each releases its own XCD.  One cross-XCD hop per episode;
+//   * device form: the device master (leader of sub-group 0) gathers every
+//     sub-group leader's flag, one per lane, and writes one release word
+//     per XCD (gout[x]) for every XCD.  Two cross-XCD hops.
+// Every WG of XCD x waits on gout[x].  The local-exchange barrier is
+// all-to-all: each member stores its in-flag and polls the other members'.
+// Flags and data are agent-scope loads and drained stores, so no level
+// needs a cache fence.  A flag that, within one launch, is stored and
+// polled only inside one XCD (the sub-group and group flags; the release
+// words of the XCD form) is stored plainly and stays dirty in that XCD's
+// L2, where the agent-scope polls find it without a memory round trip
+// (lfTreeBarrUniq at 1024 x 256: 210k -> 241k episodes/s, 253k with the
+// XCD form).  Cross-XCD flags are written through.
 __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int q,
                                                  uint32_t el, uint32_t eg,
                                                  bool local_only, uint32_t poll_q,
@@ -750,7 +774,7 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
     // global barrier in between, which every member reaches after its
     // polls here, so each poll sees exactly `el` before it moves on.
     uint32_t *lin = f.lin + (long)grp * f.per_cu * S;
-    if (tid == 0) st_rlx(lin + q * S, el);
+    if (tid == 0) st_xcd(lin + q * S, el);
     bool ok = true;
     for (int m = tid; m < f.per_cu; m += nthr)
       if (m != q) ok = ok && poll_eq(lin + m * S, el, poll_q, g);
@@ -767,7 +791,7 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
   uint32_t *sin = f.sin + sid * sw * S;
   if (m != 0) {  // member: arrive at the sub level, wait for the release
     if (tid == 0) {
-      st_rlx(sin + m * S, eg);
+      st_xcd(sin + m * S, eg);
       *s_flag = poll_eq_pipe(rel, eg, poll_q, g);
     }
     __syncthreads();
@@ -777,6 +801,24 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
   bool ok = true;
   for (int j = 1 + tid; j < ck; j += nthr) ok = ok && poll_eq(sin + j * S, eg, poll_q, g);
   if (!__syncthreads_and(ok)) return false;
+  if (f.a2a) {
+    // XCD leader: publish the XCD's arrival (written through: other XCDs
+    // poll it), gather the other leaders', release its own XCD.  A leader
+    // can publish the next episode before a slower leader has seen this
+    // one (it needs only its own XCD's arrivals), never two episodes ahead,
+    // so the gather accepts any epoch >= eg.
+    const int nx = f.nab < f.nxcd ? f.nab : f.nxcd;
+    if (tid == 0) st_rlx(f.din + (long)x * S, eg);
+    ok = true;
+    for (int y = tid; y < nx; y += nthr)
+      if (y != x) ok = ok && poll_ge(f.din + (long)y * S, eg, poll_q, g);
+    if (!__syncthreads_and(ok)) return false;
+    if (tid == 0) {
+      st_xcd(f.gout + (long)x * S, eg);  // stored and polled on this XCD only
+      if (x == 0) count_rlx(f.episodes);
+    }
+    return true;
+  }
   if (sid != 0) {  // arrive at the device level, wait for the release
     if (tid == 0) {
       st_rlx(f.din + sid * S, eg);

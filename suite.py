@@ -53,7 +53,7 @@ def c2(dev, terms):
             line = {"episodes_per_s": r.episodes / (r.roi_ms / 1e3), "episode_us": ep_us,
                     "parity_ok": bool(np.array_equal(s, want)) and r.state_ok == 1}
             if terms and name in bench.BARRIERS and not r.sync_scope_used:
-                fl = bench.barrier_floor_us(name, terms)
+                fl = bench.barrier_floor_us(name, terms, ncu)
                 line.update(floor_us=fl, frac_of_floor=fl / ep_us)
             out[f"{name}_{form}{'' if r.sync_scope_used == sc else '_fallback'}"] = line
     return out
