@@ -46,9 +46,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 # lines and raw figures are in profiles/k_barrier_atomicTreeBarrUniq_1024x256.txt.
 # Key: (prim, sync scope, WG size, numLdSt, numCSIters, num WGs, mode); mode 0
 # is the reference's copy chain, mode 1 the in-place increment the headline
-# runs (FETCH 89638.1 KiB x 1.99982 + WRITE 226471.1 KiB x 1.000).
+# runs (FETCH 89432.4 KiB x 1.99982 + WRITE 226699.0 KiB x 1.000).
 MEASURED_TRAFFIC = {("atomicTreeBarrUniq", 0, 256, 10, 1000, 1024, 0): 405270138,
-                    ("atomicTreeBarrUniq", 0, 256, 10, 1000, 1024, 1): 415468290}
+                    ("atomicTreeBarrUniq", 0, 256, 10, 1000, 1024, 1): 415280494}
 METRIC = ("acquire+release/s & barrier episodes/s at 1 GPU, batched vs plain; "
           "UTS nodes/s, p99")
 BARRIERS = ("atomicTreeBarrUniq", "atomicTreeBarrUniqLocalExch", "lfTreeBarrUniq",
