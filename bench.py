@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Benchmark driver for the synchronization hot path on one MI355X.
+"""Benchmark of the synchronization hot path on one MI355X.
 
 Headline line: barrier episodes/s of the full-co-residency grid barrier
 (atomicTreeBarrUniq, 1024 workgroups = 4 per CU x 256 CUs, 256 threads,
@@ -581,7 +581,7 @@ def cpu_baselines(prim, L, nw, wg, ncu, seconds):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {a.gpus} requested); "
+        print(f"bench.py: measures one GPU only (--gpus {a.gpus} requested); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import heterosync_amd as hs

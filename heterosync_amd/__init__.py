@@ -2,7 +2,7 @@
 
 The product is the C-ABI library libheterosync_amd.so (include/heterosync.h)
 plus the drop-in command lines under heterosync_amd/bin/.  This module is a
-thin ctypes binding to that library for tests, the benchmark driver and
+thin ctypes binding to that library for tests, the benchmark and
 Python callers.  There is no fallback: if the library is missing or cannot
 be loaded, every entry point raises.
 """

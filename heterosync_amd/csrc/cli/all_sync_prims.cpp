@@ -128,7 +128,7 @@ int main(int argc, char **argv) {
     return -1;
   }
   if (!hs_prim_supported(p)) {
-    std::fprintf(stderr, "ERROR: %s (#%d) is not part of this build\n", prim_str, p);
+    std::fprintf(stderr, "ERROR: %s (#%d) is not implemented by this library\n", prim_str, p);
     hs_close(ctx);
     return -1;
   }

@@ -86,7 +86,7 @@ const char *hs_strerror(int code) {
     case HS_EOCCUPANCY: return "workgroups cannot all be co-resident";
     case HS_ETIMEOUT: return "device watchdog fired (wait exceeded timeout)";
     case HS_ENODEV: return "no HIP device";
-    case HS_EUNSUPPORTED: return "primitive not part of this build";
+    case HS_EUNSUPPORTED: return "primitive not implemented by this library";
     case HS_ESCRATCH: return "workload needs more than the no-allocation scratch (4 MiB)";
     default: return "unknown error";
   }

@@ -49,7 +49,7 @@ enum {
   HS_EOCCUPANCY = -4,  /* workgroups cannot all be co-resident */
   HS_ETIMEOUT = -5,    /* device watchdog fired: a wait exceeded timeout_s */
   HS_ENODEV = -6,      /* no HIP device */
-  HS_EUNSUPPORTED = -7, /* primitive id without a kernel in this build (every
+  HS_EUNSUPPORTED = -7, /* primitive id without a kernel in this library (every
                           id of the reference table has one; kept for ABI
                           stability) */
   HS_ESCRATCH = -8     /* hs_syncprim_run_noalloc: the run's state does not
@@ -75,7 +75,7 @@ int hs_device_info(hs_ctx *ctx, int *num_cu, int *max_threads_per_block,
  * 43-46 reserved).  Returns -1 for unknown names. */
 int hs_prim_id(const char *name);
 const char *hs_prim_name(int id);
-/* 1 if this build implements the id (0-35 and 37-42; 36 is reserved in the
+/* 1 if this library implements the id (0-35 and 37-42; 36 is reserved in the
  * reference table), else 0. */
 int hs_prim_supported(int id);
 
