@@ -674,7 +674,14 @@ int run_once(hs_ctx *c, const hs_syncprim_cfg *cfg, int32_t *storage, long n,
   a.per_cu = per;
   a.bo.min_q = cfg->backoff_min > 0 ? cfg->backoff_min : 1;
   a.bo.max_q = cfg->backoff_max > 0 ? cfg->backoff_max : 1023;
-  a.bo.poll_q = cfg->poll_sleep < 0 ? 0 : (cfg->poll_sleep > 0 ? cfg->poll_sleep : 1);
+  // Default: one sleep quantum between polls, none for the XCD-aware tree
+  // barriers, whose waits mostly poll lines kept in their XCD's L2 (1024 x
+  // 256, numLdSt 10: lfTreeBarrUniq 3.91 -> 3.85 us per episode, the
+  // atomic tree 3.29 -> 3.28).
+  const bool tree = p <= 3;
+  a.bo.poll_q = cfg->poll_sleep < 0 ? 0
+                : cfg->poll_sleep > 0 ? cfg->poll_sleep
+                : (tree && xcd_mode) ? 0 : 1;
   a.ticks = timeout_ticks(c, cfg->timeout_s);
   const bool lx = p == 1 || p == 3 || p == 38 || p == 40 || p == 42;
   const bool srb = p >= 37 && p <= 42;

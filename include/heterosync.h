@@ -94,7 +94,8 @@ typedef struct {
   int mode;          /* 0 = copy chain (reference), 1 = in-place increment */
   int backoff_min;   /* EBO backoff start, s_sleep quanta; 0 = 1 */
   int backoff_max;   /* EBO cap mask (2^k-1); 0 = 1023 (MAX_BACKOFF-1) */
-  int poll_sleep;    /* quanta between plain polls; <0 = 0, 0 = default */
+  int poll_sleep;    /* quanta between plain polls; <0 = 0, 0 = default
+                        (1, or 0 for the XCD-aware tree barriers) */
   double timeout_s;  /* per-launch watchdog; 0 = 20 s */
   int sync_scope;    /* 0 = XCD-aware barriers where the device has several
                         XCDs (one cache writeback/invalidate per XCD and
