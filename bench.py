@@ -94,11 +94,12 @@ def probe_floor(nw, wg, L):
 def lf_xcd_form(nw, wg, ncu=256, nxcd=8):
     """Whether the XCD-aware lock-free barrier runs its XCD form (one
     sub-group per XCD, the XCD leaders all-to-all, each releasing its own
-    XCD): the library takes it when the XCD leader has a lane for each of
-    its XCD's other WGs (lf_xcd_form in hs_syncprims.hip)."""
+    XCD): the library takes it when the XCD leader's lanes can gather its
+    XCD's other WGs' flags at up to 8 flags per lane (lf_xcd_form in
+    hs_syncprims.hip)."""
     nab = min(nw, ncu)
     per = -(-nw // nab)
-    return (-(-nab // nxcd)) * per - 1 <= wg
+    return (-(-nab // nxcd)) * per - 1 <= 8 * wg
 
 
 def barrier_floor_us(name, t, ncu=256):

@@ -357,15 +357,15 @@ int sub_group_size(int grid, bool flags) {
 
 // Whether the XCD-aware lock-free barrier takes its XCD form (one
 // sub-group per XCD, the XCD leaders all-to-all, each releasing its own
-// XCD): when the leader has a lane for each of its XCD's other WGs, so its
-// gather polls one flag per lane.  Measured per episode (numLdSt 10, two
-// forms: device / XCD): 256 x 64 3.54 / 3.52 us, 512 x 256 3.76 / 3.60,
-// 1024 x 256 4.16 / 4.03, 512 x 512 4.41 / 4.17, 256 x 1024 4.79 / 4.63;
-// with several flags per lane the device form wins (1024 x 64 3.87 / 3.88,
-// 2048 x 64 4.06 / 4.39, 4096 x 64 5.53 / 6.69).
+// XCD): when the leader's lanes can gather the XCD's other WGs' flags at up
+// to kGatherMax flags per lane, all of a lane's loads in flight together.
+// Measured per episode (numLdSt 10; device form / XCD form): 256 x 64
+// 3.54 / 3.43 us, 512 x 256 3.76 / 3.35, 1024 x 256 4.16 / 3.85, 512 x 512
+// 4.41 / 4.01, 256 x 1024 4.79 / 4.38, 1024 x 64 (two flags per lane) 3.83 /
+// 3.57, 2048 x 64 (four) 4.02 / 3.86, 4096 x 64 (eight) 5.72 / 4.55.
 bool lf_xcd_form(int nab, int per, int nxcd, int wg) {
   const int nwx = (nab / nxcd + (nab % nxcd ? 1 : 0)) * per;  // most WGs on one XCD
-  return nwx - 1 <= wg;
+  return nwx - 1 <= kGatherMax * wg;
 }
 
 bool is_barrier(int p) { return p < 4 || (p >= 37 && p <= 42); }

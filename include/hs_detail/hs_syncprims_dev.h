@@ -738,6 +738,38 @@ __device__ __forceinline__ bool poll_ge(const uint32_t *p, uint32_t v,
   return true;
 }
 
+// Most flags one lane of a gathering WG polls at once (gather_eq's K).
+constexpr int kGatherMax = 8;
+
+// A lane's share of a gather: the flags first + i * step below end, i < K,
+// at base + j * stride, polled until every one equals v.  All K loads are
+// issued together each pass (a lane with fewer flags re-reads its first),
+// so a pass costs one round trip whatever K is, with no per-flag branches.
+template <int K>
+__device__ __forceinline__ bool gather_eq(const uint32_t *base, long stride, int first, int step,
+                                          int end, uint32_t v, uint32_t poll_q, Guard &g) {
+  if (first >= end) return true;
+  const uint32_t *p[K];
+  bool valid[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const int j = first + i * step;
+    valid[i] = j < end;
+    p[i] = base + (long)(valid[i] ? j : first) * stride;
+  }
+  for (;;) {
+    uint32_t w[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) w[i] = ld_rlx(p[i]);
+    bool all = true;
+#pragma unroll
+    for (int i = 0; i < K; ++i) all = all && (!valid[i] || w[i] == v);
+    if (all) return true;
+    if (!g.ok()) return false;
+    sleep_quanta(poll_q);
+  }
+}
+
 // Whole-WG call.  el = group-level epoch, eg = episode (device level).
 // The global barrier gathers arrival flags on two levels, as the tree
 // barrier counts them: the WGs of one XCD form sub-groups of up to
@@ -801,9 +833,17 @@ __device__ __forceinline__ bool xcd_flag_barrier(const XcdFlags &f, int grp, int
     __syncthreads();
     return *s_flag != 0;
   }
-  // sub-group leader: gather the members, one flag per lane
-  bool ok = true;
-  for (int j = 1 + tid; j < ck; j += nthr) ok = ok && poll_eq(sin + j * S, eg, poll_q, g);
+  // sub-group leader: gather the members, flags 1 + tid + i * nthr per lane
+  const int kpl = (ck - 1 + nthr - 1) / nthr;  // flags per lane, WG-uniform
+  bool ok;
+  if (kpl <= 1) ok = gather_eq<1>(sin, S, 1 + tid, nthr, ck, eg, poll_q, g);
+  else if (kpl <= 2) ok = gather_eq<2>(sin, S, 1 + tid, nthr, ck, eg, poll_q, g);
+  else if (kpl <= 4) ok = gather_eq<4>(sin, S, 1 + tid, nthr, ck, eg, poll_q, g);
+  else if (kpl <= kGatherMax) ok = gather_eq<kGatherMax>(sin, S, 1 + tid, nthr, ck, eg, poll_q, g);
+  else {
+    ok = true;
+    for (int j = 1 + tid; j < ck; j += nthr) ok = ok && poll_eq(sin + j * S, eg, poll_q, g);
+  }
   if (!__syncthreads_and(ok)) return false;
   if (f.a2a) {
     // XCD leader: publish the XCD's arrival (written through: other XCDs

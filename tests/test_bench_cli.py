@@ -21,15 +21,17 @@ def _bench():
 
 
 def test_lf_form_choice_follows_the_library_rule():
-    """The XCD form of the lock-free barrier runs when an XCD's leader has a
-    lane for each of the XCD's other WGs (256 CUs, 8 XCDs)."""
+    """The XCD form of the lock-free barrier runs when an XCD's leader can
+    gather the XCD's other WGs' flags at up to 8 per lane (256 CUs, 8
+    XCDs)."""
     b = _bench()
     assert b.lf_xcd_form(1024, 256)       # 128 WGs per XCD, 256 lanes
     assert b.lf_xcd_form(256, 64)         # 32 per XCD
     assert b.lf_xcd_form(512, 512)
     assert b.lf_xcd_form(5, 64)           # one WG on each of 5 XCDs
-    assert not b.lf_xcd_form(1024, 64)    # 128 per XCD, 64 lanes
-    assert not b.lf_xcd_form(4096, 64)
+    assert b.lf_xcd_form(1024, 64)        # 128 per XCD: two flags per lane
+    assert b.lf_xcd_form(4096, 64)        # 512 per XCD: eight per lane
+    assert not b.lf_xcd_form(8192, 64)    # 1024 per XCD: sixteen per lane
 
 
 def test_barrier_floor_terms():
@@ -42,5 +44,5 @@ def test_barrier_floor_terms():
     assert abs(b.barrier_floor_us("atomicTreeBarrUniqLocalExch", t) - (3.0 + 0.6 + 0.5 + 0.4)) < 1e-9
     assert abs(b.barrier_floor_us("lfTreeBarrUniq", t) - (1.5 + 0.4 + 0.5)) < 1e-9
     assert abs(b.barrier_floor_us("lfTreeBarrUniqLocalExch", t) - (3.0 + 0.6 + 0.5)) < 1e-9
-    t64 = dict(t, lanes=64)  # device form: member hop + two cross-XCD hops
+    t64 = dict(t, wgs=8192, lanes=64)  # device form: member hop + two cross-XCD hops
     assert abs(b.barrier_floor_us("lfTreeBarrUniq", t64) - (1.5 + 0.2 + 1.0)) < 1e-9
