@@ -98,6 +98,20 @@ def test_barrier_small(dev, ohelp, prim, mode, scope):
     check(dev, ohelp, prim, 2, 2 * ncu, 3, reps=2, mode=mode, wg=128, sync_scope=scope)
 
 
+@pytest.mark.parametrize("prim", BARRIER)
+@pytest.mark.parametrize("scope", [0, 1])
+def test_barrier_uneven_xcds(dev, ohelp, prim, scope):
+    """Grids that do not spread evenly over the XCDs (up to NUM_CU WGs; a
+    larger grid is a multiple of it): the XCDs hold different numbers of
+    WGs and sub-groups, so the XCD leaders gather different numbers of
+    flags.  Several flags per lane: test_barrier_small's 64-lane grids of
+    4 to 16 WGs per CU."""
+    for nw in (3, 13, 100, 203):
+        for wg in (64, 256):
+            r = check(dev, ohelp, prim, 3, nw, 40, reps=2, wg=wg, mode=1, sync_scope=scope)
+            assert r.sync_scope_used == scope
+
+
 @pytest.mark.parametrize("prim", BARRIER + SRB)
 @pytest.mark.parametrize("scope", [0, 1])
 def test_barrier_regions_return(dev, ohelp, prim, scope):
