@@ -735,8 +735,10 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
       }
       s_off[lane] = off;
       if (lane == 63) s_off[64] = off + (lane < n ? nc : 0);
-      // per-tree accounting: one lane when the batch is all one tree (the
-      // held count changes by total - n: children in, parents out)
+      // per-tree accounting: one lane when the batch is all one tree.  The
+      // popped parents leave the held count; the reference tree's children
+      // enter it here (total), a geometric tree's as they are generated
+      // (only the ones that are not leaves are stored and held)
       {
         const uint32_t slot0 = __builtin_amdgcn_readlane(slot, 0);
         const bool active = lane < n;
@@ -753,14 +755,14 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
           if (lane == 0) {
             s_nodes[slot0] += (unsigned long long)n;
             s_leaves[slot0] += (unsigned long long)__popcll(leafmask);
-            s_hold[slot0] += (int)total - n;
+            s_hold[slot0] += (TREE == 1 ? 0 : (int)total) - n;
             if (TREE == 1 && dmax > s_maxd[slot0]) s_maxd[slot0] = dmax;
             if (TREE == 0) s_lsum[slot0] += lsum;
           }
         } else if (active) {
           atomicAdd(&s_nodes[slot], 1ull);
           if (nc == 0) atomicAdd(&s_leaves[slot], 1ull);
-          atomicAdd(&s_hold[slot], (int)nc - 1);
+          atomicAdd(&s_hold[slot], (TREE == 1 ? 0 : (int)nc) - 1);
           if (TREE == 1) atomicMax(&s_maxd[slot], dep);
           if (TREE == 0 && lval) atomicAdd(&s_lsum[slot], (unsigned long long)lval);
         }
@@ -777,10 +779,20 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
       }
       phase(st_tpop);
       // ---- generate the children, 64 per round ---------------------------
-      // the newest min(total, 64) children stay in s_top (slot c - keep0)
+      // Reference tree: the newest min(total, 64) children stay in s_top
+      // (slot c - keep0), the others go to the deque.  Geometric tree: a
+      // child's own child count is known from its digest as soon as it is
+      // generated, and a leaf (at the depth limit, or drawn below the first
+      // threshold: count 0) is counted at once and never stored, so the
+      // deque and the batches hold only records with children (about a
+      // fifth of the nodes at b0 = 4).  The stored children are packed: the
+      // first 64 into s_top, the rest into the deque under them (their
+      // order within a batch does not matter), s_top slot k being deque
+      // position top - lds_n + k as always.
       const uint32_t keep = total < (uint32_t)kWave ? total : (uint32_t)kWave;
       const uint32_t keep0 = total - keep;
       uint32_t carry = 0;  // parent of the last child of the previous round
+      uint32_t stored = 0;  // geometric: children stored so far (wave-uniform)
       for (uint32_t base = 0; base < total; base += kWave) {
         ++st_rounds;
         // Parent of child c: the last parent with children whose first child
@@ -798,39 +810,80 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
         carry = (uint32_t)__builtin_amdgcn_readlane(lo, 63);
         __syncthreads();  // s_first is read before the next round clears it
         const uint32_t c = base + lane;
-        if (c < total) {
-          const uint32_t *pr = &s_pst[lo * W];
-          const long at = (top + c) & dq_mask;
-          if (TREE == 1) {
-            const uint32_t i = c - s_off[lo];
-            uint32_t ch[5];
+        const bool valid = c < total;
+        if (TREE == 1) {
+          uint32_t ch[5] = {0, 0, 0, 0, 0}, cmeta = 0;
+          bool inner = false;
+          if (valid) {
+            const uint32_t *pr = &s_pst[lo * W];
             const uint32_t par[5] = {pr[0], pr[1], pr[2], pr[3], pr[4]};
-            uts_spawn(par, i, ch);
-            if (c >= keep0) {
+            uts_spawn(par, c - s_off[lo], ch);
+            cmeta = pr[5] + 1;  // depth + 1, same slot
+            inner = meta_depth(cmeta) < s_gmx[meta_slot(cmeta)] &&
+                    (ch[4] & 0x7fffffffu) >= s_thr[0];
+          }
+          const unsigned long long im = __ballot(valid && inner);
+          const uint32_t q = stored + __builtin_amdgcn_mbcnt_hi(
+                                          (uint32_t)(im >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)im, 0u));
+          if (valid && inner) {
+            if (q < (uint32_t)kWave) {
 #pragma unroll
-              for (int w = 0; w < 5; ++w) s_top[w * kWave + (c - keep0)] = ch[w];
-              s_top[5 * kWave + (c - keep0)] = pr[5] + 1;  // depth + 1, same slot
+              for (int w = 0; w < 5; ++w) s_top[w * kWave + q] = ch[w];
+              s_top[5 * kWave + q] = cmeta;
             } else {
+              const long at = (top + (q - kWave)) & dq_mask;
 #pragma unroll
               for (int w = 0; w < 5; ++w) deq[dq_idx<W>(at, w)] = ch[w];
-              deq[dq_idx<W>(at, 5)] = pr[5] + 1;
+              deq[dq_idx<W>(at, 5)] = cmeta;
             }
-          } else {
-            const uint32_t node = pr[0];
-            const uint32_t kid =
-                (c - s_off[lo]) == 0 ? (uint32_t)a.child_a[node] : (uint32_t)a.child_b[node];
-            if (c >= keep0) {
-              s_top[c - keep0] = kid;
-              s_top[kWave + (c - keep0)] = pr[1];
+          }
+          // this round's children: the stored ones are held, the leaves done
+          const uint32_t csl = meta_slot(cmeta), sl0 = __builtin_amdgcn_readlane(csl, 0);
+          const unsigned long long lm = __ballot(valid && !inner);
+          if (__all(!valid || csl == sl0)) {
+            uint32_t dmax = 0;
+            if (lm) dmax = __builtin_amdgcn_readlane(
+                              wave_incl_scan<true>(valid && !inner ? meta_depth(cmeta) : 0), 63);
+            if (lane == 0) {
+              s_hold[sl0] += __popcll(im);
+              s_nodes[sl0] += (unsigned long long)__popcll(lm);
+              s_leaves[sl0] += (unsigned long long)__popcll(lm);
+              if (dmax > s_maxd[sl0]) s_maxd[sl0] = dmax;
+            }
+          } else if (valid) {
+            if (inner) {
+              atomicAdd(&s_hold[csl], 1);
             } else {
-              deq[dq_idx<W>(at, 0)] = kid;
-              deq[dq_idx<W>(at, 1)] = pr[1];
+              atomicAdd(&s_nodes[csl], 1ull);
+              atomicAdd(&s_leaves[csl], 1ull);
+              atomicMax(&s_maxd[csl], meta_depth(cmeta));
             }
+          }
+          st_nodes += __popcll(lm);
+          stored += (uint32_t)__popcll(im);
+        } else if (valid) {
+          const uint32_t *pr = &s_pst[lo * W];
+          const long at = (top + c) & dq_mask;
+          const uint32_t node = pr[0];
+          const uint32_t kid =
+              (c - s_off[lo]) == 0 ? (uint32_t)a.child_a[node] : (uint32_t)a.child_b[node];
+          if (c >= keep0) {
+            s_top[c - keep0] = kid;
+            s_top[kWave + (c - keep0)] = pr[1];
+          } else {
+            deq[dq_idx<W>(at, 0)] = kid;
+            deq[dq_idx<W>(at, 1)] = pr[1];
           }
         }
       }
-      top += total;
-      lds_n = (int)keep;
+      if (TREE == 1) {
+        top += stored;
+        lds_n = stored < (uint32_t)kWave ? (int)stored : kWave;
+      } else {
+        top += total;
+        lds_n = (int)keep;
+      }
       if (top - bot > (long)high_water) high_water = (uint32_t)(top - bot);
       ++batches;
       st_nodes += n;
