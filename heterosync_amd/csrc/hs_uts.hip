@@ -119,10 +119,96 @@ __device__ __forceinline__ void sha1_block(uint32_t h[5], uint32_t w[16]) {
   h[4] = 0xc3d2e1f0u + e;
 }
 
-// child i of a parent digest: SHA1(parent(20 B) || i(4 B, big endian))
-__device__ __forceinline__ void uts_spawn(const uint32_t p[5], uint32_t i, uint32_t out[5]) {
-  uint32_t w[16] = {p[0], p[1], p[2], p[3], p[4], i, 0x80000000u, 0, 0, 0, 0, 0, 0, 0, 0, 192u};
-  sha1_block(out, w);
+// Child i of a parent digest p: SHA1(p (20 B) || i (4 B, big endian)), one
+// padded block w = {p0..p4, i, 0x80000000, 0 x 8, 192}.  Split in two: the
+// part that depends on the parent alone is computed once per parent and
+// shared by its children (about 45 of the block's ~600 instructions).
+//  - Rounds 0-4 read only p0..p4.  Round 5 adds w5 = i to a sum of the
+//    round-4 state: P0 = rotl(a,5) + Ch(b,c,d) + e + K, so a6 = P0 + i, and
+//    b6..e6 = a5, rotl(b5,30), c5, d5 (P1..P4).
+//  - Of the schedule, w16, w17, w18, w20, w23 and w26 do not involve w5 (by
+//    w[t] = rotl1(w[t-3] ^ w[t-8] ^ w[t-14] ^ w[t-16]) over the constant
+//    words); w19 = rotl1(w16 ^ w11 ^ w5 ^ w3) is rotl1(P8 ^ i).  Every other
+//    w[t], t >= 21, reads only w5..w15 and schedule words, so a child needs
+//    P0..P11 and i, never p itself.
+constexpr int kPre = 12;
+__device__ __forceinline__ void sha1_prefix(const uint32_t p[5], uint32_t P[kPre]) {
+  uint32_t a = 0x67452301u, b = 0xefcdab89u, c = 0x98badcfeu, d = 0x10325476u,
+           e = 0xc3d2e1f0u;
+#pragma unroll
+  for (int t = 0; t < 5; ++t) {
+    const uint32_t tmp = rotl(a, 5) + lop3_ch(b, c, d) + e + 0x5a827999u + p[t];
+    e = d;
+    d = c;
+    c = rotl(b, 30);
+    b = a;
+    a = tmp;
+  }
+  P[0] = rotl(a, 5) + lop3_ch(b, c, d) + e + 0x5a827999u;
+  P[1] = a;
+  P[2] = rotl(b, 30);
+  P[3] = c;
+  P[4] = d;
+  // the message with w5 = 0: the words below do not read it
+  uint32_t m[27] = {p[0], p[1], p[2], p[3], p[4], 0, 0x80000000u, 0, 0, 0, 0, 0, 0, 0, 0, 192u};
+#pragma unroll
+  for (int t = 16; t < 27; ++t) m[t] = rotl(lop3_xor(m[t - 3], m[t - 8], m[t - 14]) ^ m[t - 16], 1);
+  P[5] = m[16];
+  P[6] = m[17];
+  P[7] = m[18];
+  P[8] = lop3_xor(m[16], m[11], m[3]);  // w19 without its w5 term, before the rotate
+  P[9] = m[20];
+  P[10] = m[23];
+  P[11] = m[26];
+}
+
+__device__ __forceinline__ void sha1_child(const uint32_t P[kPre], uint32_t i, uint32_t h[5]) {
+  // message words 6..15 (w0..w4 are only read through P); the ring slot of
+  // w[t] is t & 15
+  uint32_t w[16] = {0, 0, 0, 0, 0, i, 0x80000000u, 0, 0, 0, 0, 0, 0, 0, 0, 192u};
+  uint32_t a = P[0] + i, b = P[1], c = P[2], d = P[3], e = P[4];
+#pragma unroll
+  for (int t = 6; t < 80; ++t) {
+    uint32_t wt;
+    if (t < 16) {
+      wt = w[t];
+    } else {
+      if (t == 16) wt = P[5];
+      else if (t == 17) wt = P[6];
+      else if (t == 18) wt = P[7];
+      else if (t == 19) wt = rotl(P[8] ^ i, 1);
+      else if (t == 20) wt = P[9];
+      else if (t == 23) wt = P[10];
+      else if (t == 26) wt = P[11];
+      else wt = rotl(lop3_xor(w[(t - 3) & 15], w[(t - 8) & 15], w[(t - 14) & 15]) ^ w[t & 15], 1);
+      w[t & 15] = wt;
+    }
+    uint32_t f, k;
+    if (t < 20) {
+      f = lop3_ch(b, c, d);
+      k = 0x5a827999u;
+    } else if (t < 40) {
+      f = lop3_xor(b, c, d);
+      k = 0x6ed9eba1u;
+    } else if (t < 60) {
+      f = lop3_maj(b, c, d);
+      k = 0x8f1bbcdcu;
+    } else {
+      f = lop3_xor(b, c, d);
+      k = 0xca62c1d6u;
+    }
+    const uint32_t tmp = rotl(a, 5) + f + e + k + wt;
+    e = d;
+    d = c;
+    c = rotl(b, 30);
+    b = a;
+    a = tmp;
+  }
+  h[0] = 0x67452301u + a;
+  h[1] = 0xefcdab89u + b;
+  h[2] = 0x98badcfeu + c;
+  h[3] = 0x10325476u + d;
+  h[4] = 0xc3d2e1f0u + e;
 }
 
 // root: SHA1(16 zero bytes || seed(4 B, big endian))
@@ -335,6 +421,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   constexpr int W = Rec<TREE>::W;
   __shared__ uint32_t s_pst[W * kWave];   // parent records of the batch
   __shared__ uint32_t s_top[W * kWave];   // newest children, word-major, not in the deque
+  __shared__ uint32_t s_pre[TREE == 1 ? kPre * kWave : 1];  // per parent: sha1_prefix
   __shared__ uint32_t s_off[kWave + 1];   // exclusive scan of child counts
   __shared__ uint32_t s_first[kWave];     // per round: parent + 1 at its first child
   __shared__ int s_hold[kMaxSlots];       // nodes of each slot held by this WG
@@ -672,8 +759,16 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
 #pragma unroll
           for (int w = 0; w < W; ++w) r[w] = deq[dq_idx<W>(at, w)];
         }
+        if (TREE == 1) {  // the children read only the meta word and the prefix
+          s_pst[lane * W + W - 1] = r[W - 1];
+          uint32_t P[kPre];
+          sha1_prefix(r, P);
 #pragma unroll
-        for (int w = 0; w < W; ++w) s_pst[lane * W + w] = r[w];
+          for (int j = 0; j < kPre; ++j) s_pre[lane * kPre + j] = P[j];
+        } else {
+#pragma unroll
+          for (int w = 0; w < W; ++w) s_pst[lane * W + w] = r[w];
+        }
         meta = r[W - 1];
         slot = meta_slot(meta);
         if (TREE == 1) {
@@ -815,10 +910,11 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
           uint32_t ch[5] = {0, 0, 0, 0, 0}, cmeta = 0;
           bool inner = false;
           if (valid) {
-            const uint32_t *pr = &s_pst[lo * W];
-            const uint32_t par[5] = {pr[0], pr[1], pr[2], pr[3], pr[4]};
-            uts_spawn(par, c - s_off[lo], ch);
-            cmeta = pr[5] + 1;  // depth + 1, same slot
+            uint32_t P[kPre];
+#pragma unroll
+            for (int j = 0; j < kPre; ++j) P[j] = s_pre[lo * kPre + j];
+            sha1_child(P, c - s_off[lo], ch);
+            cmeta = s_pst[lo * W + W - 1] + 1;  // depth + 1, same slot
             inner = meta_depth(cmeta) < s_gmx[meta_slot(cmeta)] &&
                     (ch[4] & 0x7fffffffu) >= s_thr[0];
           }
