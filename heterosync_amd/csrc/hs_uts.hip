@@ -832,17 +832,22 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
       if (lane == 63) s_off[64] = off + (lane < n ? nc : 0);
       // per-tree accounting: one lane when the batch is all one tree.  The
       // popped parents leave the held count; the reference tree's children
-      // enter it here (total), a geometric tree's as they are generated
-      // (only the ones that are not leaves are stored and held)
+      // enter it here (total), a geometric tree's after they are generated
+      // (only the ones that are not leaves are stored and held).  A child
+      // is one deeper than its parent, so a geometric batch's depth maximum
+      // takes dep + 1 for every parent with children: its leaves are covered
+      // without looking at them, the others are counted again when popped.
+      const uint32_t slot0 = __builtin_amdgcn_readlane(slot, 0);
+      const bool batch_uni = __all(lane >= n || slot == slot0);
       {
-        const uint32_t slot0 = __builtin_amdgcn_readlane(slot, 0);
         const bool active = lane < n;
-        if (__all(!active || slot == slot0)) {
+        if (batch_uni) {
           const unsigned long long leafmask = __ballot(active && nc == 0);
           uint32_t dmax = 0;
           unsigned long long lsum = active ? lval : 0;
           if (TREE == 1) {
-            dmax = __builtin_amdgcn_readlane(wave_incl_scan<true>(active ? dep : 0), 63);
+            dmax = __builtin_amdgcn_readlane(
+                wave_incl_scan<true>(active ? dep + (nc > 0 ? 1u : 0u) : 0u), 63);
           } else {
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
@@ -858,7 +863,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
           atomicAdd(&s_nodes[slot], 1ull);
           if (nc == 0) atomicAdd(&s_leaves[slot], 1ull);
           atomicAdd(&s_hold[slot], (TREE == 1 ? 0 : (int)nc) - 1);
-          if (TREE == 1) atomicMax(&s_maxd[slot], dep);
+          if (TREE == 1) atomicMax(&s_maxd[slot], dep + (nc > 0 ? 1u : 0u));
           if (TREE == 0 && lval) atomicAdd(&s_lsum[slot], (unsigned long long)lval);
         }
       }
@@ -934,29 +939,17 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
               deq[dq_idx<W>(at, 5)] = cmeta;
             }
           }
-          // this round's children: the stored ones are held, the leaves done
-          const uint32_t csl = meta_slot(cmeta), sl0 = __builtin_amdgcn_readlane(csl, 0);
-          const unsigned long long lm = __ballot(valid && !inner);
-          if (__all(!valid || csl == sl0)) {
-            uint32_t dmax = 0;
-            if (lm) dmax = __builtin_amdgcn_readlane(
-                              wave_incl_scan<true>(valid && !inner ? meta_depth(cmeta) : 0), 63);
-            if (lane == 0) {
-              s_hold[sl0] += __popcll(im);
-              s_nodes[sl0] += (unsigned long long)__popcll(lm);
-              s_leaves[sl0] += (unsigned long long)__popcll(lm);
-              if (dmax > s_maxd[sl0]) s_maxd[sl0] = dmax;
-            }
-          } else if (valid) {
+          // the stored children are held, the leaves done: counted after the
+          // batch when it is all one tree, else here, per child
+          if (!batch_uni && valid) {
+            const uint32_t csl = meta_slot(cmeta);
             if (inner) {
               atomicAdd(&s_hold[csl], 1);
             } else {
               atomicAdd(&s_nodes[csl], 1ull);
               atomicAdd(&s_leaves[csl], 1ull);
-              atomicMax(&s_maxd[csl], meta_depth(cmeta));
             }
           }
-          st_nodes += __popcll(lm);
           stored += (uint32_t)__popcll(im);
         } else if (valid) {
           const uint32_t *pr = &s_pst[lo * W];
@@ -974,6 +967,12 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
         }
       }
       if (TREE == 1) {
+        if (batch_uni && lane == 0) {
+          s_hold[slot0] += (int)stored;
+          s_nodes[slot0] += (unsigned long long)(total - stored);
+          s_leaves[slot0] += (unsigned long long)(total - stored);
+        }
+        st_nodes += total - stored;
         top += stored;
         lds_n = stored < (uint32_t)kWave ? (int)stored : kWave;
       } else {
