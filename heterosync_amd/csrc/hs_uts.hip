@@ -1095,10 +1095,10 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   }
 }
 
-// One tree per launch.  The register budget admits four waves per SIMD
-// (at most 128 VGPRs; the kernel needs 116, no spills): a grid of that size
-// hides the record loads of one wave behind the SHA-1 work of the others on
-// big trees.
+// One tree per launch.  The register budget admits four waves per SIMD (at
+// most 128 VGPRs; the geometric kernel needs 86, no spills, and 9.7 KB of
+// LDS, 16 WGs per CU); big trees run three per SIMD by default, whose record
+// loads hide behind the SHA-1 work of the other waves.
 template <int TREE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 k_uts(UtsArgs a) {
@@ -1112,8 +1112,10 @@ k_uts_stats(UtsArgs a) {
   uts_worker<TREE, false, true>(a);
 }
 
-// The persistent serving loop runs two waves per SIMD (small trees gain
-// nothing from a bigger grid), so it keeps its full register allocation.
+// The persistent serving loop keeps its full register allocation (about 150
+// VGPRs: three waves per SIMD).  Held to 128 (four per SIMD, a few spills) it
+// served the 256-tree depth 6-14 mix no faster (61.0-61.1 against 61.4-62.1
+// G nodes/s on one MI355X).
 __global__ void __launch_bounds__(64) k_uts_serve(UtsArgs a) { uts_worker<1, true, false>(a); }
 
 // One tree with host worker threads taking part (hs_uts_*_run_cpu).
@@ -1276,18 +1278,19 @@ struct Engine {
     return ctl[0] ? HS_ETIMEOUT : HS_OK;
   }
 
-  // One geometric tree: 8 one-wave WGs per CU, or as many as can be resident
-  // (four per SIMD) when the tree is expected to be big.  Measured on
-  // MI355X (b0 = 4): the full grid wins from about 1e9 nodes (depth 15:
-  // 41.8 G nodes/s at 4096 WGs, 38.7 at 3072, 35.1 at 2048) and loses
-  // below (depth 12: 11.8 vs 14.7 G nodes/s at 3072 vs 2048), where more
-  // WGs only add idle polling.
+  // One geometric tree: 8 one-wave WGs per CU (two per SIMD), or 12 (three
+  // per SIMD, if that many can be resident) when the tree is expected to be
+  // big.  Measured on MI355X (b0 = 4), kernel time at 2048 / 3072 / 4096
+  // WGs: depth 15 (7.85e9 nodes) 138.0 / 129.7 / 129.7 ms, depth 14
+  // (1.96e9) 37.4 / 36.6 / 38.5-39.6 ms, depth 13 (4.9e8) 11.5 / 12.6 /
+  // 13.9 ms, depth 12 4.6 / 6.4 / 6.7 ms: below about 1e9 nodes more WGs
+  // only add idle polling.
   static int default_blocks(hs_ctx *c, double b0, int gen_mx) {
     if (std::pow(b0, (double)gen_mx) < 2.5e8) return 8 * c->num_cu;
     int fit = 0;
     if (coresident_wgs(c, (const void *)k_uts<1>, 64, &fit) != HS_OK || fit <= 0)
       return 8 * c->num_cu;
-    return fit;
+    return std::min(fit, 12 * c->num_cu);
   }
 };
 
