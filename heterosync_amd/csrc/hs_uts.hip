@@ -132,6 +132,11 @@ __device__ __forceinline__ void sha1_block(uint32_t h[5], uint32_t w[16]) {
 //    w[t], t >= 21, reads only w5..w15 and schedule words, so a child needs
 //    P0..P11 and i, never p itself.
 constexpr int kPre = 12;
+// A parent's LDS record for its children: the prefix, then the child meta
+// (depth + 1, same slot), the leaf threshold (a child is a leaf iff its draw
+// is below it) and the index of its first child in the batch; 16 words, so
+// a child finds it with one shift and four 16-byte loads.
+constexpr int kPreStride = 16;
 __device__ __forceinline__ void sha1_prefix(const uint32_t p[5], uint32_t P[kPre]) {
   uint32_t a = 0x67452301u, b = 0xefcdab89u, c = 0x98badcfeu, d = 0x10325476u,
            e = 0xc3d2e1f0u;
@@ -419,9 +424,11 @@ __device__ void dispatch(const UtsArgs &a, Guard &g) {
 template <int TREE, bool SERVE, bool STATS, bool CPU = false>
 __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
   constexpr int W = Rec<TREE>::W;
-  __shared__ uint32_t s_pst[W * kWave];   // parent records of the batch
+  // parent records of the batch (the reference tree; the serving
+  // dispatcher's admission uses two words of it)
+  __shared__ uint32_t s_pst[TREE == 1 ? 2 : W * kWave];
   __shared__ uint32_t s_top[W * kWave];   // newest children, word-major, not in the deque
-  __shared__ uint32_t s_pre[TREE == 1 ? kPre * kWave : 1];  // per parent: sha1_prefix
+  __shared__ __align__(16) uint32_t s_pre[TREE == 1 ? kPreStride * kWave : 1];
   __shared__ uint32_t s_off[kWave + 1];   // exclusive scan of child counts
   __shared__ uint32_t s_first[kWave];     // per round: parent + 1 at its first child
   __shared__ int s_hold[kMaxSlots];       // nodes of each slot held by this WG
@@ -748,7 +755,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
       // under the deque bound, so a breadth-first batch is never cut
       const bool bfs = held < kGrow && held + (long)kWave * max_children<TREE>(a) <= dq_soft;
       if (bfs) flush_top();  // pops from the bottom; this batch refills s_top
-      uint32_t nc = 0, meta = 0, slot = 0, dep = 0, lval = 0;
+      uint32_t nc = 0, meta = 0, slot = 0, dep = 0, lval = 0, cthr = 0;
       if (lane < n) {
         const long at = (bfs ? bot + lane : top - 1 - lane) & dq_mask;
         uint32_t r[W];
@@ -759,12 +766,11 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
 #pragma unroll
           for (int w = 0; w < W; ++w) r[w] = deq[dq_idx<W>(at, w)];
         }
-        if (TREE == 1) {  // the children read only the meta word and the prefix
-          s_pst[lane * W + W - 1] = r[W - 1];
+        if (TREE == 1) {  // the children read only the prefix record
           uint32_t P[kPre];
           sha1_prefix(r, P);
 #pragma unroll
-          for (int j = 0; j < kPre; ++j) s_pre[lane * kPre + j] = P[j];
+          for (int j = 0; j < kPre; ++j) s_pre[lane * kPreStride + j] = P[j];
         } else {
 #pragma unroll
           for (int w = 0; w < W; ++w) s_pst[lane * W + w] = r[w];
@@ -786,6 +792,9 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
             nc = k;
           }
           dep = depth;
+          // a child at the depth limit is a leaf whatever its draw, and no
+          // draw (< 2^31) reaches 0xffffffff
+          cthr = depth + 1 < s_gmx[slot] ? s_thr[0] : 0xffffffffu;
         } else {
           const int32_t ca = a.child_a[r[0]];
           nc = ca == 0 ? 0 : 2;
@@ -828,8 +837,15 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
         off = wave_excl_scan(lane < n ? nc : 0, &total);
         if (bound_cut) ++cut_batches;
       }
-      s_off[lane] = off;
-      if (lane == 63) s_off[64] = off + (lane < n ? nc : 0);
+      if (TREE == 1) {
+        s_pre[lane * kPreStride + kPre] = meta + 1;
+        s_pre[lane * kPreStride + kPre + 1] = cthr;
+        s_pre[lane * kPreStride + kPre + 2] = off;
+        s_pre[lane * kPreStride + kPre + 3] = 0;
+      } else {
+        s_off[lane] = off;
+        if (lane == 63) s_off[64] = off + (lane < n ? nc : 0);
+      }
       // per-tree accounting: one lane when the batch is all one tree.  The
       // popped parents leave the held count; the reference tree's children
       // enter it here (total), a geometric tree's after they are generated
@@ -912,27 +928,34 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
         const uint32_t c = base + lane;
         const bool valid = c < total;
         if (TREE == 1) {
-          uint32_t ch[5] = {0, 0, 0, 0, 0}, cmeta = 0;
-          bool inner = false;
-          if (valid) {
-            uint32_t P[kPre];
+          // every lane hashes (lo is a parent of this batch either way; a
+          // lane past the batch's children hashes a value nobody keeps)
+          uint32_t P[kPreStride];
+          const uint4 *pp = reinterpret_cast<const uint4 *>(&s_pre[lo * kPreStride]);
 #pragma unroll
-            for (int j = 0; j < kPre; ++j) P[j] = s_pre[lo * kPre + j];
-            sha1_child(P, c - s_off[lo], ch);
-            cmeta = s_pst[lo * W + W - 1] + 1;  // depth + 1, same slot
-            inner = meta_depth(cmeta) < s_gmx[meta_slot(cmeta)] &&
-                    (ch[4] & 0x7fffffffu) >= s_thr[0];
+          for (int j = 0; j < kPreStride / 4; ++j) {
+            const uint4 t = pp[j];
+            P[4 * j] = t.x;
+            P[4 * j + 1] = t.y;
+            P[4 * j + 2] = t.z;
+            P[4 * j + 3] = t.w;
           }
-          const unsigned long long im = __ballot(valid && inner);
+          uint32_t ch[5];
+          sha1_child(P, c - P[kPre + 2], ch);
+          const uint32_t cmeta = P[kPre];
+          const bool inner = valid && (ch[4] & 0x7fffffffu) >= P[kPre + 1];
+          const unsigned long long im = __ballot(inner);
           const uint32_t q = stored + __builtin_amdgcn_mbcnt_hi(
                                           (uint32_t)(im >> 32),
                                           __builtin_amdgcn_mbcnt_lo((uint32_t)im, 0u));
-          if (valid && inner) {
-            if (q < (uint32_t)kWave) {
+          const uint32_t nst = (uint32_t)__popcll(im);
+          if (inner && q < (uint32_t)kWave) {
 #pragma unroll
-              for (int w = 0; w < 5; ++w) s_top[w * kWave + q] = ch[w];
-              s_top[5 * kWave + q] = cmeta;
-            } else {
+            for (int w = 0; w < 5; ++w) s_top[w * kWave + q] = ch[w];
+            s_top[5 * kWave + q] = cmeta;
+          }
+          if (stored + nst > (uint32_t)kWave) {  // wave-uniform, and rare: past the LDS top
+            if (inner && q >= (uint32_t)kWave) {
               const long at = (top + (q - kWave)) & dq_mask;
 #pragma unroll
               for (int w = 0; w < 5; ++w) deq[dq_idx<W>(at, w)] = ch[w];
@@ -950,7 +973,7 @@ __device__ __forceinline__ void uts_worker(const UtsArgs &a) {
               atomicAdd(&s_leaves[csl], 1ull);
             }
           }
-          stored += (uint32_t)__popcll(im);
+          stored += nst;
         } else if (valid) {
           const uint32_t *pr = &s_pst[lo * W];
           const long at = (top + c) & dq_mask;
