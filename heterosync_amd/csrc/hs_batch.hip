@@ -607,14 +607,26 @@ BatchFn pick_batch(const hs_batch_cfg *cfg) {
 // time go in blocks as well (their writers hand over in cohorts); the wider
 // ones stay round robin over the free slots, since their readers run
 // concurrently only on different workers.
-std::vector<int> place_vwgs(const Batch &B, int P, int vpw, bool blocks) {
+// The blocks fill the workers XCD by XCD (w = x, x + nxcd, x + 2 nxcd, ...:
+// the workers a round-robin dispatch puts on one XCD), so an instance's
+// workers share an XCD unless it straddles the end of one XCD's slots, and
+// its lock hand-offs and the lines they pass stay within one L2 and one
+// XCD's path to memory.  On the seeded 64-instance mix: 311-319 ms per
+// launch against 343-368 ms with the blocks over consecutive workers (one
+// per XCD in turn).  Packing the instances onto the XCDs by weight
+// (heaviest first, each on the least loaded XCD that holds it) ran 429-436
+// ms.
+std::vector<int> place_vwgs(const Batch &B, int P, int vpw, bool blocks, int nxcd) {
   std::vector<int> map((size_t)P * vpw, -1);
   if (!blocks) {
     for (int v = 0; v < B.nv; ++v) map[(size_t)(v % P) * vpw + v / P] = v;
     return map;
   }
-  std::vector<int> fill(P, 0);
-  int w = 0;
+  std::vector<int> fill(P, 0), order;
+  if (nxcd < 1 || nxcd > P) nxcd = 1;
+  for (int x = 0; x < nxcd; ++x)
+    for (int w = x; w < P; w += nxcd) order.push_back(w);
+  int wi = 0;
   // in blocks: every mutex, and the semaphores whose readers enter one or
   // two at a time (spinSem1/2, spinSemEBO1/2), whose writers hand over in
   // cohorts like the mutexes
@@ -622,8 +634,9 @@ std::vector<int> place_vwgs(const Batch &B, int P, int vpw, bool blocks) {
   for (const BInst &I : B.h) {
     if (!blocked(I.prim)) continue;
     for (int k = 0; k < I.nw; ++k) {
-      while (w < P && fill[w] == vpw) ++w;
-      if (w == P) return {};
+      while (wi < P && fill[order[wi]] == vpw) ++wi;
+      if (wi == P) return {};
+      const int w = order[wi];
       map[(size_t)w * vpw + fill[w]++] = I.vbase + k;
     }
   }
@@ -658,7 +671,7 @@ int batch_launch(hs_ctx *c, Batch &B, int n, const hs_batch_cfg *cfg, int reps, 
   if (a.vpw > kMaxVpw) return HS_EINVAL;
   int rc;
   {
-    const std::vector<int> map = place_vwgs(B, P, a.vpw, cfg->cohort > 1);
+    const std::vector<int> map = place_vwgs(B, P, a.vpw, cfg->cohort > 1, num_xcds(c));
     if (map.empty()) return HS_EINVAL;
     int *d_map = nullptr;
     if ((rc = B.ar.alloc(&d_map, map.size()))) return rc;
@@ -775,8 +788,13 @@ extern "C" int hs_batch_autotune(hs_ctx *c, const hs_batch_instance *in, int n,
   if (!c || !in || n < 1 || !cfg) return HS_EINVAL;
   if (cfg->mode != 0 && cfg->mode != 1) return HS_EINVAL;
   HS_HIP(hipSetDevice(c->device));
-  // calibration copy: iterations capped so each trial takes milliseconds
-  std::vector<hs_batch_instance> cal(in, in + n);
+  // calibration copy: iterations capped so each trial takes milliseconds.
+  // The cohort is timed on the batch itself: its holds grow with the
+  // sections each WG still owes, and no capped copy ranked it right on the
+  // seeded 64-instance mix (up to 1000 iterations), where a cohort of 8 ran
+  // 395-436 ms per launch against 311-319 ms for 64: capped at 20 the two
+  // timed alike, capped at 200 the cohort of 8 won (144-152 ms).
+  std::vector<hs_batch_instance> cal(in, in + n), full(in, in + n);
   for (auto &x : cal) x.num_cs_iters = std::min(x.num_cs_iters, 20);
   double best = 1e30;
   hs_batch_cfg best_cfg = *cfg;
@@ -786,9 +804,10 @@ extern "C" int hs_batch_autotune(hs_ctx *c, const hs_batch_instance *in, int n,
   best_cfg.cohort = 0;
   int best_wpc = 8;
   // One calibration launch of t; *ms < 0 when its workers cannot all be resident.
-  auto trial = [&](const hs_batch_cfg &t, double *ms) -> int {
+  auto trial_on = [&](const std::vector<hs_batch_instance> &set, const hs_batch_cfg &t,
+                      double *ms) -> int {
     Batch B;
-    int rc = batch_prepare(c, cal.data(), n, 1, B, nullptr);
+    int rc = batch_prepare(c, set.data(), n, 1, B, nullptr);
     if (rc) return rc;
     const int P = default_workers(c, &t, B.nv);
     rc = check_workers(c, &t, P, B.nv);
@@ -802,6 +821,7 @@ extern "C" int hs_batch_autotune(hs_ctx *c, const hs_batch_instance *in, int n,
     HS_HIP(hipMemcpy(&ab, B.d_ctl, 4, hipMemcpyDeviceToHost));
     return ab ? HS_ETIMEOUT : HS_OK;
   };
+  auto trial = [&](const hs_batch_cfg &t, double *ms) { return trial_on(cal, t, ms); };
   // stage 1, 64-lane workers starting backoff at 1 quantum (the reference's
   // NUM_WIS_PER_WG and backoff start, main.hip.cpp:9, hipLocksMutexEBO.h:26):
   // waves per CU x poll interval x EBO cap
@@ -847,14 +867,16 @@ extern "C" int hs_batch_autotune(hs_ctx *c, const hs_batch_instance *in, int n,
         best_cfg = t;
       }
     }
-  // last, the spin / EBO mutex cohort at the winning knobs
+  // last, the spin / EBO mutex cohort at the winning knobs, on the batch
+  // itself
   {
     const hs_batch_cfg base2 = best_cfg;
-    for (int coh : {8, 64}) {
+    best = 1e30;
+    for (int coh : {0, 8, 64}) {
       hs_batch_cfg t = base2;
       t.cohort = coh;
       double ms = 0;
-      const int rc = trial(t, &ms);
+      const int rc = trial_on(full, t, &ms);
       if (rc) return rc;
       if (ms >= 0 && ms < best) {
         best = ms;

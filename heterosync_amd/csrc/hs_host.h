@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "../../include/heterosync.h"
@@ -88,6 +89,12 @@ class EventSet {
 inline uint64_t timeout_ticks(const hs_ctx *c, double s) {
   if (s <= 0) s = 20.0;
   return (uint64_t)(s * c->wall_hz);
+}
+
+// XCDs of the device: gfx950 (MI355X) / gfx942 (MI300X) have 32 CUs per XCD.
+inline int num_xcds(const hs_ctx *c) {
+  if (std::strncmp(c->arch, "gfx950", 6) && std::strncmp(c->arch, "gfx942", 6)) return 1;
+  return c->num_cu >= 64 && c->num_cu % 32 == 0 ? c->num_cu / 32 : 1;
 }
 
 // Looks up the device address of the no-allocation scratch array for the

@@ -418,12 +418,6 @@ uint32_t sem_count_of(int p) {
   return 0;
 }
 
-// XCDs of the device: gfx950 (MI355X) / gfx942 (MI300X) have 32 CUs per XCD.
-int num_xcds(const hs_ctx *c) {
-  if (std::strncmp(c->arch, "gfx950", 6) && std::strncmp(c->arch, "gfx942", 6)) return 1;
-  return c->num_cu >= 64 && c->num_cu % 32 == 0 ? c->num_cu / 32 : 1;
-}
-
 // The no-allocation form's buffers: the caller's device storage, and the
 // per-device scratch array (see hs_syncprim_run_noalloc).
 struct NoAlloc {

@@ -334,10 +334,13 @@ int hs_batch_run(hs_ctx *ctx, const hs_batch_instance *inst, int n,
  * {2,4,8,16} x poll interval {0,1,4} x EBO cap I_max {63,255,1023} with
  * 64-lane workers, then, at the winning waves per CU, worker width
  * {64,128,256} x backoff start I_min {1,4,16} x attempt form (probe 0,
- * 1), and last the mutex cohort {off, 8, 64} at the winning knobs.
- * Writes the fastest wg_size / persistent_wgs / poll_sleep / backoff_min /
- * backoff_max / probe / cohort into *cfg (grids that cannot be co-resident
- * are skipped). */
+ * 1), and last the mutex cohort {off, 8, 64} at the winning knobs on the
+ * batch itself, one launch each (a cohort's holds grow with the sections
+ * each WG still owes, which a capped copy does not show).  Writes the
+ * fastest wg_size / persistent_wgs / poll_sleep / backoff_min / backoff_max
+ * / probe / cohort into *cfg (grids that cannot be co-resident are
+ * skipped), and the chosen configuration's time for one launch of the
+ * batch into *best_ms. */
 int hs_batch_autotune(hs_ctx *ctx, const hs_batch_instance *inst, int n,
                       hs_batch_cfg *cfg, double *best_ms);
 
