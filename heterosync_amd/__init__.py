@@ -372,8 +372,9 @@ class Device:
     # ---- batched ---------------------------------------------------------
     def batch_autotune(self, instances, reps=10, mode=0, use_graph=1, timeout_s=20.0):
         """Returns the tuned knobs as a dict: wg_size, persistent_wgs,
-        poll_sleep, backoff_min, backoff_max, probe, cohort, and best_ms (the
-        winning calibration launch).  batch(**{k: v for k in TUNED}) runs them."""
+        poll_sleep, backoff_min, backoff_max, probe, cohort, and best_ms (one
+        launch of the whole batch in the chosen configuration, from the
+        cohort stage).  batch(**{k: v for k in TUNED}) runs them."""
         n = len(instances)
         inst = (BatchInstance * n)(*[BatchInstance(*i) for i in instances])
         cfg = BatchCfg(64, reps, mode, 0, 0, 0, use_graph, 0, timeout_s, 0, 0)
