@@ -45,10 +45,13 @@ def test_uts_geo_trees(dev, oracle, seed, d, blocks):
     assert (r.nodes, r.leaves, r.max_depth) == want
 
 
-@pytest.mark.parametrize("seed,b0,d", [(11, 2.0, 16), (12, 8.0, 6), (13, 3.5, 9)])
+@pytest.mark.parametrize("seed,b0,d", [(11, 2.0, 16), (12, 8.0, 6), (13, 3.5, 9),
+                                       (24, 20.0, 5), (22, 40.0, 4)])
 def test_uts_geo_other_branching(dev, oracle, seed, b0, d):
     """Other expected branching factors: the child-count threshold table is
-    built per b0."""
+    built per b0.  At b0 = 20 and 40 a parent has up to 100 children, so a
+    batch's records with children overflow the 64 kept in LDS and spill to
+    the deque in most generate rounds."""
     want = geo_count(oracle, seed, b0, d)
     r = dev.uts_geo(seed, b0, d, 0)
     assert (r.nodes, r.leaves, r.max_depth) == want
