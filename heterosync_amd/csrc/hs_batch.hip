@@ -789,13 +789,15 @@ extern "C" int hs_batch_autotune(hs_ctx *c, const hs_batch_instance *in, int n,
   if (cfg->mode != 0 && cfg->mode != 1) return HS_EINVAL;
   HS_HIP(hipSetDevice(c->device));
   // calibration copy: iterations capped so each trial takes milliseconds.
-  // The cohort is timed on the batch itself: its holds grow with the
-  // sections each WG still owes, and no capped copy ranked it right on the
+  // The cohort is timed on the batch itself (iterations capped at 2000, so
+  // that tuning a very long batch stays bounded): its holds grow with the
+  // sections each WG still owes, and no short copy ranked it right on the
   // seeded 64-instance mix (up to 1000 iterations), where a cohort of 8 ran
   // 395-436 ms per launch against 311-319 ms for 64: capped at 20 the two
   // timed alike, capped at 200 the cohort of 8 won (144-152 ms).
   std::vector<hs_batch_instance> cal(in, in + n), full(in, in + n);
   for (auto &x : cal) x.num_cs_iters = std::min(x.num_cs_iters, 20);
+  for (auto &x : full) x.num_cs_iters = std::min(x.num_cs_iters, 2000);
   double best = 1e30;
   hs_batch_cfg best_cfg = *cfg;
   best_cfg.wg_size = 64;

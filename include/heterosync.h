@@ -335,8 +335,9 @@ int hs_batch_run(hs_ctx *ctx, const hs_batch_instance *inst, int n,
  * 64-lane workers, then, at the winning waves per CU, worker width
  * {64,128,256} x backoff start I_min {1,4,16} x attempt form (probe 0,
  * 1), and last the mutex cohort {off, 8, 64} at the winning knobs on the
- * batch itself, one launch each (a cohort's holds grow with the sections
- * each WG still owes, which a capped copy does not show).  Writes the
+ * batch itself (iterations capped at 2000), one launch each (a cohort's
+ * holds grow with the sections each WG still owes, which a short copy does
+ * not show).  Writes the
  * fastest wg_size / persistent_wgs / poll_sleep / backoff_min / backoff_max
  * / probe / cohort into *cfg (grids that cannot be co-resident are
  * skipped), and the chosen configuration's time for one launch of the
