@@ -69,7 +69,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--only", default="",
-                    help="comma list of extras to run (barriers,c1,c3,c5,uts_cpu); default all")
+                    help="comma list of extras to run (barriers,c1,c3,c5,uts,uts_cpu); "
+                         "default all")
     ap.add_argument("--no-probe", action="store_true",
                     help="skip the floor probe (a child process); the roofline then has no "
                          "floor (for runs under a profiler)")
@@ -439,6 +440,39 @@ def extra_c5(dev, slots=64, loads=(0.8,), n=256, seed=9, depth=(6, 14)):
     return out
 
 
+# VALU issue peak of the GPU: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per
+# wave64 instruction, in wave-instructions per second.
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
+# The geometric kernel's VALU lane-slots per node on the depth-15 tree below
+# (SQ_INSTS_VALU x 64 / nodes; profiles/k_uts_geometric_seed7_depth15.txt).
+UTS_LANE_SLOTS_PER_NODE = 625
+
+
+def extra_uts_tree(dev, seed=7, depth=15, runs=3):
+    """One large geometric tree (b0 = 4, depth limit 15: 7.85e9 nodes) by the
+    single-tree kernel at its default grid, against the VALU issue bound.
+    Parity: every run's counts, the single-tree kernel's at a smaller grid
+    (2048 WGs) and the serving kernel's (the tree as its only request) all
+    agree."""
+    dev.uts_geo(1, 4.0, 4)
+    rs = [dev.uts_geo(seed, 4.0, depth) for _ in range(runs)]
+    alt = dev.uts_geo(seed, 4.0, depth, 2048)
+    resp, _ = dev.uts_serve([(seed, depth, 4.0, 0.0)], slots=1, timeout_s=120.0)
+    key = (rs[0].nodes, rs[0].leaves, rs[0].max_depth)
+    ok = all((r.nodes, r.leaves, r.max_depth) == key for r in rs + [alt] + list(resp))
+    ms = sorted(r.kernel_ms for r in rs)
+    med = ms[len(ms) // 2]
+    bound = VALU_ISSUE_PER_S * 64 / UTS_LANE_SLOTS_PER_NODE
+    rate = rs[0].nodes / (med / 1e3)
+    return {"workload": f"geometric tree seed {seed}, b0=4, depth {depth}: {rs[0].nodes} nodes, "
+                        f"{rs[0].leaves} leaves",
+            "kernel_ms": [round(m, 3) for m in ms], "nodes_per_s": round(rate, 1),
+            "roofline": {"bound": "valu-issue", "peak_nodes_per_s": round(bound, 1),
+                         "frac": round(rate / bound, 4),
+                         "lane_slots_per_node": UTS_LANE_SLOTS_PER_NODE},
+            "parity_ok": bool(ok)}
+
+
 def extra_uts_host_workers(dev, seed=7, depth=13, threads=(0, 4, 16)):
     """One large geometric tree (b0 = 4) searched by the GPU alone and with
     host worker threads taking part (each served by one GPU WG through a
@@ -587,7 +621,8 @@ def main():
         sys.exit(2)
     import heterosync_amd as hs
 
-    only = set(filter(None, a.only.split(","))) or {"barriers", "c1", "c3", "c5", "uts_cpu"}
+    only = set(filter(None, a.only.split(","))) or {"barriers", "c1", "c3", "c5", "uts",
+                                                     "uts_cpu"}
     # the probe runs first, in its own process, before this one opens the device
     ncu_guess = 256
     floor_terms = None if a.no_probe else probe_floor(4 * ncu_guess, a.wg, a.num_ldst)
@@ -667,6 +702,8 @@ def main():
             extras["c3_batched"] = extra_c3(dev, hs, floor_terms)
         if "c5" in only:
             extras["c5_serving"] = extra_c5(dev)
+        if "uts" in only:
+            extras["uts_tree"] = extra_uts_tree(dev)
         if "uts_cpu" in only:
             extras["uts_host_workers"] = extra_uts_host_workers(dev)
         out["extras"] = extras
