@@ -694,18 +694,28 @@ def main():
     }
     if not a.no_extras:
         extras = {}
+
+        def run_extra(key, fn, *args):
+            # an extra that fails is reported in its own entry; the headline
+            # line above is printed either way
+            try:
+                extras[key] = fn(*args)
+            except Exception as e:  # noqa: BLE001 -- any failure of one extra
+                extras[key] = {"error": f"{type(e).__name__}: {e}", "parity_ok": False}
+                print(f"bench.py: extra {key} failed: {e}", file=sys.stderr)
+
         if "barriers" in only:
-            extras["barriers"] = extra_barriers(dev, hs, a, nw, floor_terms)
+            run_extra("barriers", extra_barriers, dev, hs, a, nw, floor_terms)
         if "c1" in only:
-            extras["c1_faMutex"] = extra_c1(dev, hs, floor_terms)
+            run_extra("c1_faMutex", extra_c1, dev, hs, floor_terms)
         if "c3" in only:
-            extras["c3_batched"] = extra_c3(dev, hs, floor_terms)
+            run_extra("c3_batched", extra_c3, dev, hs, floor_terms)
         if "c5" in only:
-            extras["c5_serving"] = extra_c5(dev)
+            run_extra("c5_serving", extra_c5, dev)
         if "uts" in only:
-            extras["uts_tree"] = extra_uts_tree(dev)
+            run_extra("uts_tree", extra_uts_tree, dev)
         if "uts_cpu" in only:
-            extras["uts_host_workers"] = extra_uts_host_workers(dev)
+            run_extra("uts_host_workers", extra_uts_host_workers, dev)
         out["extras"] = extras
     dev.close()
     if not a.no_cpu:
