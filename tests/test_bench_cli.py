@@ -46,3 +46,43 @@ def test_barrier_floor_terms():
     assert abs(b.barrier_floor_us("lfTreeBarrUniqLocalExch", t) - (3.0 + 0.6 + 0.5)) < 1e-9
     t64 = dict(t, wgs=8192, lanes=64)  # device form: member hop + two cross-XCD hops
     assert abs(b.barrier_floor_us("lfTreeBarrUniq", t64) - (1.5 + 0.2 + 1.0)) < 1e-9
+
+
+class _R:
+    def __init__(self, nodes, leaves, depth, ms=0.0):
+        self.nodes, self.leaves, self.max_depth, self.kernel_ms = nodes, leaves, depth, ms
+
+
+class _FakeDev:
+    """Stands in for the device: fixed counts per call site."""
+
+    def __init__(self, alt=None, serve=None, runs_ms=(130.0, 128.0, 129.0)):
+        self.calls = []
+        self.ms = list(runs_ms)
+        self.alt, self.serve = alt, serve
+
+    def uts_geo(self, seed, b0, depth, blocks=0):
+        self.calls.append((seed, b0, depth, blocks))
+        if depth == 4:
+            return _R(100, 80, 4)
+        if blocks:
+            return self.alt or _R(1000, 800, 15)
+        return _R(1000, 800, 15, self.ms.pop(0))
+
+    def uts_serve(self, reqs, slots=0, timeout_s=0.0):
+        return [self.serve or _R(1000, 800, 15)], None
+
+
+def test_uts_tree_extra_parity_and_rate():
+    """The single-tree line agrees only when every run, the second grid and
+    the serving kernel give the same counts; its rate is the median run's."""
+    b = _bench()
+    out = b.extra_uts_tree(_FakeDev())
+    assert out["parity_ok"] is True
+    assert out["kernel_ms"] == [128.0, 129.0, 130.0]
+    assert abs(out["nodes_per_s"] - 1000 / 0.129) <= 0.05  # rounded to 0.1
+    peak = out["roofline"]["peak_nodes_per_s"]
+    assert abs(out["roofline"]["frac"] - round(out["nodes_per_s"] / peak, 4)) < 1e-9
+    assert not b.extra_uts_tree(_FakeDev(alt=_R(1000, 799, 15)))["parity_ok"]
+    assert not b.extra_uts_tree(_FakeDev(serve=_R(1001, 800, 15)))["parity_ok"]
+    assert not b.extra_uts_tree(_FakeDev(serve=_R(1000, 800, 14)))["parity_ok"]
